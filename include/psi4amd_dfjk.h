@@ -1,0 +1,117 @@
+/*
+ * psi4amd_dfjk.h -- MI355X density-fitted J/K engine behind a C ABI.
+ *
+ * Drop-in for psi4's in-core DF-JK: the JK object API of
+ * psi4/src/psi4/libfock/jk.h:232-589 with the MemDFJK algorithm
+ * (libfock/MemDFJK.cc:56-162) and DFHelper::build_JK
+ * (lib3index/dfhelper.h:329-331, dfhelper.cc:3015-3377).  Each entry point
+ * names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns DFJK_OK (0) or a non-zero status; no C++
+ *     exception crosses this boundary.  dfjk_last_error() gives the message
+ *     of the last failure on the calling thread (psi4 raises PsiException,
+ *     libpsi4util/exception.h:54, which its pybind layer turns into RuntimeError).
+ *   - Matrices are row-major FP64 (psi4 Matrix::pointer()[0] layout).
+ *   - Host-pointer entry points copy through device buffers owned by the
+ *     handle; *_device entry points take device pointers and never copy to the host.
+ *   - Calls on one handle are not reentrant; separate handles are independent.
+ *   - All device work of a handle runs on one HIP stream (dfjk_set_stream).
+ */
+#ifndef PSI4AMD_DFJK_H
+#define PSI4AMD_DFJK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "psi4amd_ints.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dfjk_handle dfjk_handle;
+
+enum dfjk_status {
+    DFJK_OK = 0,
+    DFJK_ERR_ARG = 1,      /* bad argument or shape (psi4: "JK: ... mismatch") */
+    DFJK_ERR_STATE = 2,    /* call out of lifecycle order */
+    DFJK_ERR_MEMORY = 3,   /* memory budget or device allocation */
+    DFJK_ERR_DEVICE = 4,   /* HIP runtime error */
+    DFJK_ERR_INTERNAL = 5
+};
+
+enum dfjk_precision {
+    DFJK_PREC_FP64 = 0,    /* FP64 MFMA everywhere */
+    DFJK_PREC_FP32 = 1,    /* K: FP32 MFMA, FP32 accumulate; J: FP64 */
+    DFJK_PREC_BF16X3 = 2   /* K: 3-term BF16 split products, FP32 accumulate; J: FP64 */
+};
+
+enum dfjk_fitting {
+    DFJK_FIT_CHOLESKY = 0, /* B = L^-1 (P|mn), J = L L^T (host FP64 Cholesky) */
+    DFJK_FIT_EIGEN = 1     /* B = J^-1/2 (P|mn) with psi4's condition cut (libmints/matrix.cc:2370-2424) */
+};
+
+const char* dfjk_last_error(void);
+
+/* JK::build_JK(primary, aux, ...) for MEM_DF (libfock/jk.cc:206-230) -> MemDFJK ctor (MemDFJK.cc:56-64). */
+int dfjk_create(dfjk_handle** out, const p4a_basis* primary, const p4a_basis* aux);
+/* Tensor-level handle: nbf x nbf x naux fitted tensor supplied by dfjk_set_B_* or dfjk_fill_synthetic
+ * (DFHelper with a caller-owned Ppq_, dfhelper.h:51-597). */
+int dfjk_create_tensor(dfjk_handle** out, int nbf, int naux);
+int dfjk_destroy(dfjk_handle* h);
+
+/* Knobs (jk.h:375-420; MemDFJK::set_condition jk.h:1181-1187). */
+int dfjk_set_memory(dfjk_handle* h, size_t doubles);       /* JK::set_memory: device workspace budget */
+int dfjk_set_cutoff(dfjk_handle* h, double cutoff);        /* JK::set_cutoff: Schwarz cutoff (default 1e-12) */
+int dfjk_set_condition(dfjk_handle* h, double condition);  /* MemDFJK::set_condition (default 1e-10) */
+int dfjk_set_do_J(dfjk_handle* h, int do_J);               /* JK::set_do_J */
+int dfjk_set_do_K(dfjk_handle* h, int do_K);               /* JK::set_do_K */
+int dfjk_set_omp_nthread(dfjk_handle* h, int n);           /* JK::set_omp_nthread: host integral threads */
+int dfjk_set_precision(dfjk_handle* h, int precision);     /* enum dfjk_precision (no psi4 counterpart) */
+int dfjk_set_fitting(dfjk_handle* h, int fitting);         /* enum dfjk_fitting */
+int dfjk_set_stream(dfjk_handle* h, void* hip_stream);     /* NULL: the handle's own stream */
+int dfjk_set_profiling(dfjk_handle* h, int on);            /* per-kernel HIP-event timing */
+
+/* JK::memory_estimate (MemDFJK.cc:65-69): doubles of device memory this handle needs. */
+int dfjk_memory_estimate(dfjk_handle* h, size_t* doubles);
+/* JK::initialize -> MemDFJK::preiterations -> DFHelper::initialize (dfhelper.cc:149-215):
+ * Schwarz screening, metric, host FP64 factorisation, (P|mn), GPU fit into HBM. */
+int dfjk_initialize(dfjk_handle* h);
+/* Tensor-level inputs: dense B [naux][nbf][nbf] (host or device, must be symmetric in m,n). */
+int dfjk_set_B_dense(dfjk_handle* h, const double* B, int on_device);
+/* Seeded synthetic B_Qmn = s_Q exp(-|m-n|/64) (G_Qmn+G_Qnm)/2, s_Q=(Q+1)^-1/2, G~N(0,1) (Philox4x32-10). */
+int dfjk_fill_synthetic(dfjk_handle* h, uint64_t seed);
+/* Read back B_Q for Q in [q0,q1) as dense [q1-q0][nbf][nbf] host array (tests). */
+int dfjk_get_B_dense(dfjk_handle* h, int q0, int q1, double* out);
+
+/* JK::compute (jk.cc:595-681) -> MemDFJK::compute_JK (MemDFJK.cc:97-111) -> DFHelper::build_JK.
+ * n density pairs; Cr == NULL means lr_symmetric (C_right = C_left, jk.cc:597-602).
+ * Cl[i], Cr[i]: nbf x ncol[i] row-major.  D/J/K outputs: nbf x nbf (any may be NULL).
+ * J and K are overwritten unless accumulate != 0 (then J += ..., K += ...). */
+int dfjk_compute(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                 double* const* D, double* const* J, double* const* K, int accumulate);
+int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                        double* const* D, double* const* J, double* const* K, int accumulate);
+/* Restrict the auxiliary index range used by compute (J and K are additive over Q). */
+int dfjk_set_aux_range(dfjk_handle* h, int q0, int q1);
+
+/* Adjoint of E = tr(D(2J - K)), D = C C^T (no psi4 counterpart; the DF gradient terms of
+ * scfgrad/jk_grad.cc:294-832 contract the same quantities):
+ *   dEdC = 4 (2J - K) C   (nbf x nocc),   dEdB[Q][m][n] = 4 d_Q D_mn - 2 (D B_Q D)_mn.
+ * Device pointers; dEdB may be NULL. */
+int dfjk_energy_adjoint_device(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB);
+
+/* Timing of the last compute when profiling is on: milliseconds per kernel class and launch counts. */
+int dfjk_get_timings(dfjk_handle* h, double* ms_j, double* ms_ht, double* ms_kgemm, double* ms_other, int* n_ht,
+                     int* n_kgemm);
+/* Sizes: nbf, naux, stored tile pairs, significant function pairs. */
+int dfjk_get_info(dfjk_handle* h, int* nbf, int* naux, int* ntile_pairs, int64_t* nsig_pairs);
+
+/* JK::finalize (jk.cc) -> postiterations: frees the workspace, keeps B. */
+int dfjk_finalize(dfjk_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

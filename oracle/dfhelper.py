@@ -1,0 +1,255 @@
+"""ORACLE (test infrastructure): FP64 restatement of psi4's DFHelper J/K path.
+
+Each function cites the reference lines it restates (paths relative to
+psi4/src/psi4/).  Integrals (the Libint2 role) come from the host integral
+engine ``psi4_amd.ints``; everything downstream of the integrals -- screening,
+metric power, fitting, J and K -- is restated here independently of the HIP
+path it checks.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+# --------------------------------------------------------------------- sparsity
+@dataclass
+class Sparsity:
+    nbf: int
+    naux: int
+    shell_mask: np.ndarray           # bool [nshell, nshell]
+    schwarz_fun_index: np.ndarray    # int  [nbf, nbf], 1-based rank in row, 0 = dropped
+    small_skips: np.ndarray          # [nbf+1]
+    big_skips: np.ndarray            # [nbf+1]
+    symm_small_skips: np.ndarray     # [nbf]
+    symm_ignored_columns: np.ndarray  # [nbf]
+    symm_big_skips: np.ndarray       # [nbf+1]
+    tolerance: float
+
+    @property
+    def fun_mask(self):
+        return self.schwarz_fun_index > 0
+
+    def nsig_pairs(self):
+        return int(self.small_skips[self.nbf])
+
+
+def prepare_sparsity(fun_max, shell_max, max_val, cutoff, naux):
+    """lib3index/dfhelper.cc:299-420 (DFHelper::prepare_sparsity), after the
+    (mn|mn) loop: tolerance = cutoff^2 / max, keep pairs with value >= tolerance."""
+    nbf = fun_max.shape[0]
+    tol = cutoff * cutoff / max_val
+    shell_mask = shell_max >= tol                                   # :372
+    fun_index = np.zeros((nbf, nbf), dtype=np.int64)
+    small = np.zeros(nbf + 1, dtype=np.int64)
+    for i in range(nbf):                                            # :375-386
+        count = 0
+        for j in range(nbf):
+            if fun_max[i, j] >= tol:
+                count += 1
+                fun_index[i, j] = count
+        small[i] = count
+    big = np.zeros(nbf + 1, dtype=np.int64)                         # :388-397
+    for j in range(nbf):
+        big[j + 1] = big[j] + small[j] * naux
+    small[nbf] = small[:nbf].sum()
+    ssmall = np.zeros(nbf, dtype=np.int64)                          # :399-410
+    signored = np.zeros(nbf, dtype=np.int64)
+    for i in range(nbf):
+        sig = fun_index[i] > 0
+        ssmall[i] = int(sig[i:].sum())
+        signored[i] = int(sig[:i].sum())
+    sbig = np.zeros(nbf + 1, dtype=np.int64)                        # :412-415
+    for i in range(1, nbf + 1):
+        sbig[i] = sbig[i - 1] + ssmall[i - 1] * naux
+    return Sparsity(nbf, naux, shell_mask, fun_index, small, big, ssmall, signored, sbig, tol)
+
+
+def memory_estimate(sp: Sparsity, qshell_max: int, nthreads: int = 1, do_wK=False, wcombine=False):
+    """lib3index/dfhelper.cc:216-236 (DFHelper::AO_core) size in doubles, as
+    returned by MemDFJK::memory_estimate (libfock/MemDFJK.cc:65-69)."""
+    nbf, naux = sp.nbf, sp.naux
+    big = int(sp.big_skips[nbf])
+    size = (2 * big if wcombine else 3 * big) if do_wK else big
+    size += naux * naux
+    size += nthreads * nbf * nbf
+    size += 3 * nbf * nbf * qshell_max
+    return size
+
+
+# --------------------------------------------------------------------- metric
+def metric_power(J, power=-0.5, condition=1e-10):
+    """libmints/matrix.cc:2370-2424 (Matrix::power): eigendecomposition, eigenvalues
+    with |a| < condition*max|a| dropped for negative powers, non-finite results
+    zeroed.  Returns (J^power, number kept, eigenvalues)."""
+    a, V = np.linalg.eigh(J)
+    max_a = max(abs(a[-1]), abs(a[0]))
+    ap = np.empty_like(a)
+    remain = 0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i, ai in enumerate(a):
+            if power < 0.0 and abs(ai) < condition * max_a:
+                ap[i] = 0.0
+            else:
+                v = ai ** power if ai >= 0 else np.float64(ai) ** power
+                if np.isfinite(v):
+                    ap[i] = v
+                    remain += 1
+                else:
+                    ap[i] = 0.0
+    return (V * ap) @ V.T, remain, a
+
+
+# --------------------------------------------------------------------- B build
+def build_ppq(Pmn, metp, sp: Sparsity):
+    """Fitted tensor in psi4's pQq layout (dfhelper.cc:1284-1347 gathers the
+    symmetric q>=p part of (P|pq); :1653-1678 contracts it with the metric power
+    and copies the upper triangle to the lower one).  Pmn is dense [naux,nbf,nbf].
+    Returns the flat Ppq_ array of size big_skips[nbf]."""
+    nbf, naux = sp.nbf, sp.naux
+    out = np.zeros(int(sp.big_skips[nbf]))
+    mask = sp.fun_mask
+    for p in range(nbf):
+        si = int(sp.small_skips[p])
+        cols = np.nonzero(mask[p, p:])[0] + p        # significant q >= p
+        jump = int(sp.symm_ignored_columns[p])
+        blk = out[sp.big_skips[p]:sp.big_skips[p + 1]].reshape(naux, si)
+        blk[:, jump:jump + len(cols)] = metp @ Pmn[:, p, cols]
+    for p in range(nbf):                             # upper -> lower copy
+        for q in range(p + 1, nbf):
+            if mask[p, q]:
+                ind_lo = sp.big_skips[q] + np.arange(naux) * sp.small_skips[q] + sp.schwarz_fun_index[q, p] - 1
+                ind_up = sp.big_skips[p] + np.arange(naux) * sp.small_skips[p] + sp.schwarz_fun_index[p, q] - 1
+                out[ind_lo] = out[ind_up]
+    return out
+
+
+def ppq_row(Ppq, sp: Sparsity, k):
+    """Row block k of the pQq tensor: [naux, small_skips[k]]."""
+    return Ppq[sp.big_skips[k]:sp.big_skips[k + 1]].reshape(sp.naux, int(sp.small_skips[k]))
+
+
+def ppq_to_dense(Ppq, sp: Sparsity):
+    """B_Qmn dense [naux, nbf, nbf] with dropped pairs as zeros."""
+    B = np.zeros((sp.naux, sp.nbf, sp.nbf))
+    for k in range(sp.nbf):
+        cols = np.nonzero(sp.fun_mask[k])[0]
+        B[:, k, cols] = ppq_row(Ppq, sp, k)
+    return B
+
+
+# --------------------------------------------------------------------- J and K
+def compute_J_symm(Ppq, sp: Sparsity, D, J):
+    """dfhelper.cc:3162-3222: d_Q over m >= k with doubled off-diagonal D, then
+    the pruned upper-triangle J unpacked into both triangles (accumulates)."""
+    nbf, naux = sp.nbf, sp.naux
+    T1 = np.zeros(naux)
+    mask = sp.fun_mask
+    for k in range(nbf):
+        cols = np.nonzero(mask[k, k:])[0] + k
+        dbuf = np.where(cols == k, D[k, cols], 2.0 * D[k, cols])
+        skip = int(sp.symm_ignored_columns[k])
+        blk = ppq_row(Ppq, sp, k)[:, skip:skip + len(cols)]
+        T1 += blk @ dbuf
+    for k in range(nbf):
+        cols = np.nonzero(mask[k, k:])[0] + k
+        skip = int(sp.symm_ignored_columns[k])
+        blk = ppq_row(Ppq, sp, k)[:, skip:skip + len(cols)]
+        T2 = blk.T @ T1
+        # the first symmetric column is the diagonal (dfhelper.cc:3211 assumes it exists)
+        J[k, k] += T2[0]
+        off = cols[1:]
+        J[k, off] += T2[1:]
+        J[off, k] += T2[1:]
+
+
+def compute_J(Ppq, sp: Sparsity, D, J):
+    """dfhelper.cc:3230-3285: non-symmetric J (accumulates)."""
+    nbf, naux = sp.nbf, sp.naux
+    T1 = np.zeros(naux)
+    mask = sp.fun_mask
+    for k in range(nbf):
+        cols = np.nonzero(mask[k])[0]
+        T1 += ppq_row(Ppq, sp, k) @ D[k, cols]
+    for k in range(nbf):
+        cols = np.nonzero(mask[k])[0]
+        J[k, cols] += ppq_row(Ppq, sp, k).T @ T1
+
+
+def first_transform_pQq(Ppq, sp: Sparsity, C):
+    """dfhelper.cc:2162-2186: T[k][Q][i] = sum_{m sig} B_Q,km C[m,i]."""
+    nbf, naux = sp.nbf, sp.naux
+    nocc = C.shape[1]
+    T = np.empty((nbf, naux, nocc))
+    mask = sp.fun_mask
+    for k in range(nbf):
+        cols = np.nonzero(mask[k])[0]
+        T[k] = ppq_row(Ppq, sp, k) @ C[cols]
+    return T
+
+
+def compute_K(Ppq, sp: Sparsity, Cl, Cr, K, lr_symmetric):
+    """dfhelper.cc:3350-3377: K += T1 T2^T with T as [nbf, naux*nocc]."""
+    nocc = Cl.shape[1]
+    if nocc == 0:
+        return
+    T1 = first_transform_pQq(Ppq, sp, Cl).reshape(sp.nbf, -1)
+    T2 = T1 if lr_symmetric else first_transform_pQq(Ppq, sp, Cr).reshape(sp.nbf, -1)
+    K += T1 @ T2.T
+
+
+def build_JK(Ppq, sp: Sparsity, C_left, C_right=None, do_J=True, do_K=True):
+    """JK::compute (libfock/jk.cc:595-681) + MemDFJK::compute_JK (MemDFJK.cc:97-111)
+    + DFHelper::compute_JK (dfhelper.cc:3044-3161) with one Q block.
+    C_right None  =>  lr_symmetric (jk.cc:597-602).  Returns (D, J, K) lists."""
+    lr_symmetric = C_right is None
+    if lr_symmetric:
+        C_right = C_left
+    if len(C_left) != len(C_right):
+        raise ValueError("JK: C_left/C_right count mismatch")
+    Ds, Js, Ks = [], [], []
+    nbf = sp.nbf
+    for Cl, Cr in zip(C_left, C_right):
+        if Cl.shape[1] != Cr.shape[1]:
+            raise ValueError("JK: C_left/C_right MO zip index size mismatch!")   # jk.cc:613
+        D = Cl @ Cr.T                                                            # jk.cc:314-354
+        J = np.zeros((nbf, nbf))
+        K = np.zeros((nbf, nbf))
+        if do_J:
+            (compute_J_symm if lr_symmetric else compute_J)(Ppq, sp, D, J)
+        if do_K:
+            compute_K(Ppq, sp, Cl, Cr, K, lr_symmetric)
+        Ds.append(D)
+        Js.append(J)
+        Ks.append(K)
+    return Ds, Js, Ks
+
+
+# --------------------------------------------------------------------- driver
+class MemDFJKOracle:
+    """MemDFJK restated end to end: preiterations (MemDFJK.cc:71-96 ->
+    DFHelper::initialize, dfhelper.cc:149-215) then per-call build_JK."""
+
+    def __init__(self, primary, aux, cutoff=1e-12, condition=1e-10, nthreads=None):
+        from psi4_amd import ints  # the Libint2 stand-in (host integral engine)
+        self.primary, self.aux = primary, aux
+        self.cutoff, self.condition = cutoff, condition
+        self.nbf, self.naux = primary.nbf(), aux.nbf()
+        fm, sm, mx = ints.schwarz(primary, nthreads)
+        self.sp = prepare_sparsity(fm, sm, mx, cutoff, self.naux)
+        self.metric = ints.metric(aux, nthreads)
+        self.metp, self.n_kept, self.metric_eigs = metric_power(self.metric, -0.5, condition)
+        self.Pmn = ints.three_index(primary, aux, nthreads)
+        # dropped pairs contribute nothing (psi4 never computes them)
+        self.Pmn[:, ~self.sp.fun_mask] = 0.0
+        self.Ppq = build_ppq(self.Pmn, self.metp, self.sp)
+
+    def memory_estimate(self, nthreads=1):
+        return memory_estimate(self.sp, self.aux.max_nfunc(), nthreads)
+
+    def dense_B(self):
+        return ppq_to_dense(self.Ppq, self.sp)
+
+    def compute(self, C_left, C_right=None, do_J=True, do_K=True):
+        return build_JK(self.Ppq, self.sp, C_left, C_right, do_J, do_K)

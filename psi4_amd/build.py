@@ -1,0 +1,81 @@
+"""In-tree build of the native libraries (gfx950).
+
+  libp4a_ints.so       host integral engine + host FP64 linear algebra (g++, OpenMP)
+  libpsi4amd_dfjk.so   HIP kernels + host orchestration + the C ABI (hipcc, gfx950)
+  _core*.so            pybind11 module over the C ABI (g++)
+
+Run ``python -m psi4_amd.build`` or ``__graft_entry__.build()``.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+INC = os.path.join(ROOT, "include")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ARCH = os.environ.get("PSI4AMD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 instructions are kept out of every code object
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+HOST_FLAGS = ["-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-fopenmp"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} ... {cmd[-1]}")
+    return r
+
+
+def build_ints(verbose=False):
+    os.makedirs(LIB, exist_ok=True)
+    srcs = [os.path.join(CSRC, "ints", f) for f in ("ints.cpp", "ints_capi.cpp", "hostlinalg.cpp")]
+    out = os.path.join(LIB, "libp4a_ints.so")
+    _run(["g++", *HOST_FLAGS, "-shared", f"-I{INC}", *srcs, "-o", out + ".tmp"], verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_dfjk(verbose=False):
+    srcs = [os.path.join(CSRC, "dfjk", f) for f in ("kernels.hip", "dfjk.hip")]
+    out = os.path.join(LIB, "libpsi4amd_dfjk.so")
+    _run([os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, "-shared", f"-I{INC}", *srcs,
+          f"-L{LIB}", "-lp4a_ints", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"], verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_pybind(verbose=False):
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(HERE, "_core" + suffix)
+    src = os.path.join(CSRC, "pybind", "core.cpp")
+    pyinc = sysconfig.get_paths()["include"]
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INC}", f"-I{pybind11.get_include()}", f"-I{pyinc}",
+          f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__", src, f"-L{LIB}", "-lpsi4amd_dfjk", "-lp4a_ints",
+          "-Wl,-rpath,$ORIGIN/lib", "-o", out + ".tmp"], verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build(verbose=False):
+    build_ints(verbose)
+    build_dfjk(verbose)
+    build_pybind(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose=True)

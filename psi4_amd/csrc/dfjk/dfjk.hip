@@ -1,0 +1,856 @@
+// MI355X DF-JK engine: host orchestration and the C ABI (include/psi4amd_dfjk.h).
+//
+// Lifecycle mirrors psi4's JK object (libfock/jk.h:232-589) with the MemDFJK
+// algorithm: create (JK::build_JK, jk.cc:206-230) -> knobs -> initialize
+// (MemDFJK::preiterations -> DFHelper::initialize, dfhelper.cc:149-215) ->
+// repeated compute (JK::compute, jk.cc:595-681 -> DFHelper::build_JK,
+// dfhelper.cc:3015-3377) -> finalize.
+#include <hip/hip_runtime.h>
+#include <thread>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../ints/basis_build.hpp"
+#include "../ints/ints.hpp"
+#include "../ints/hostlinalg.hpp"
+#include "kernels.hpp"
+#include "psi4amd_dfjk.h"
+
+namespace p4a {
+namespace {
+
+thread_local std::string g_err;
+
+struct StatusError : std::runtime_error {
+    int code;
+    StatusError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw StatusError(code, msg); }
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return DFJK_OK;
+    } catch (const StatusError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return DFJK_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return DFJK_ERR_INTERNAL;
+    }
+}
+
+void hcheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) fail(DFJK_ERR_MEMORY, std::string(what) + ": device out of memory");
+        fail(DFJK_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// Growable device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* get(size_t count) {
+        const size_t need = std::max<size_t>(count * sizeof(T), 256);
+        if (need > bytes) {
+            release();
+            hcheck(hipMalloc(&p, need), "hipMalloc");
+            bytes = need;
+        }
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+struct EventTimer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    struct Rec {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hcheck(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    hipEvent_t begin() {
+        if (!on) return nullptr;
+        hipEvent_t e = take();
+        hcheck(hipEventRecord(e, s), "hipEventRecord");
+        return e;
+    }
+    void end(int cls, hipEvent_t a) {
+        if (!on) return;
+        hipEvent_t b = take();
+        hcheck(hipEventRecord(b, s), "hipEventRecord");
+        recs.push_back({cls, a, b});
+    }
+    ~EventTimer() {
+        for (auto& r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+enum TimerClass { T_J = 0, T_HT = 1, T_KGEMM = 2, T_OTHER = 3 };
+
+}  // namespace
+}  // namespace p4a
+
+using namespace p4a;
+
+struct dfjk_handle {
+    // configuration
+    bool tensor_mode = false;
+    Basis prim, aux;
+    int nbf = 0, naux = 0;
+    size_t memory_doubles = 0;  // 0: no explicit budget
+    double cutoff = 1e-12, condition = 1e-10;
+    bool do_J = true, do_K = true;
+    int nthreads = 1;
+    int precision = DFJK_PREC_FP64;
+    int fitting = DFJK_FIT_CHOLESKY;
+    bool initialized = false;
+    int aux_q0 = 0, aux_q1 = 0;
+    // stream
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    // screening and tile geometry
+    std::vector<uint8_t> fun_mask;  // nbf*nbf
+    std::vector<uint8_t> shell_mask;
+    int64_t nsig = 0;
+    int nT = 0, nbfp = 0, ntp = 0;
+    std::vector<int> tidx;
+    std::vector<int2> tiles;
+    DevBuf d_tidx, d_tiles;
+    // output 128-tile lists
+    int nB = 0;
+    DevBuf d_otiles_sym, d_otiles_all, d_otile_of_sym, d_otile_of_all;
+    std::vector<int2> otiles_sym, otiles_all;
+    std::vector<int> otile_of_sym, otile_of_all;
+    int ntile_sym = 0, ntile_all = 0;
+    // resident tensor
+    DevBuf dB;
+    // workspace
+    DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G;
+    // timing
+    EventTimer timer;
+    double t_ms[4] = {0, 0, 0, 0};
+    int n_ht = 0, n_kgemm = 0;
+
+    TileGeom geom() const {
+        TileGeom g;
+        g.nbf = nbf;
+        g.nbfp = nbfp;
+        g.nT = nT;
+        g.ntp = ntp;
+        g.d_tidx = d_tidx.as<int>();
+        g.d_tiles = d_tiles.as<int2>();
+        return g;
+    }
+};
+
+namespace {
+
+void require_state(dfjk_handle* h, bool init_needed) {
+    if (!h) fail(DFJK_ERR_ARG, "null handle");
+    if (init_needed && !h->initialized) fail(DFJK_ERR_STATE, "dfjk: call initialize() before compute()");
+}
+
+// The handle's own stream is created on first device use, so that creating a
+// handle, setting knobs and asking for the memory estimate need no device.
+void ensure_stream(dfjk_handle* h) {
+    if (!h->own_stream) hcheck(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (!h->stream) h->stream = h->own_stream;
+    h->timer.s = h->stream;
+}
+
+// Tile geometry from the function-pair mask (tile-granular screening).
+void build_tiles(dfjk_handle* h) {
+    const int n = h->nbf;
+    h->nbfp = round_up(n, TILE);
+    h->nT = h->nbfp / TILE;
+    const int nT = h->nT;
+    h->tidx.assign(static_cast<size_t>(nT) * nT, -1);
+    h->tiles.clear();
+    for (int I = 0; I < nT; ++I)
+        for (int J = I; J < nT; ++J) {
+            bool any = false;
+            for (int m = I * TILE; m < std::min(n, (I + 1) * TILE) && !any; ++m)
+                for (int q = J * TILE; q < std::min(n, (J + 1) * TILE); ++q)
+                    if (h->fun_mask[static_cast<size_t>(m) * n + q]) {
+                        any = true;
+                        break;
+                    }
+            if (any) {
+                h->tidx[I * nT + J] = static_cast<int>(h->tiles.size());
+                h->tiles.push_back(make_int2(I, J));
+            }
+        }
+    h->ntp = static_cast<int>(h->tiles.size());
+    if (h->ntp == 0) fail(DFJK_ERR_ARG, "no significant basis-function pairs");
+    // 128x128 output tiles for K/D products
+    h->nB = (n + 127) / 128;
+    const int nB = h->nB;
+    std::vector<int2> sym, all;
+    std::vector<int> of_sym(static_cast<size_t>(nB) * nB, -1), of_all(static_cast<size_t>(nB) * nB, -1);
+    for (int I = 0; I < nB; ++I)
+        for (int J = 0; J < nB; ++J) {
+            of_all[I * nB + J] = static_cast<int>(all.size());
+            all.push_back(make_int2(I, J));
+            if (I <= J) {
+                of_sym[I * nB + J] = static_cast<int>(sym.size());
+                sym.push_back(make_int2(I, J));
+            }
+        }
+    h->ntile_sym = static_cast<int>(sym.size());
+    h->ntile_all = static_cast<int>(all.size());
+    h->otiles_sym = sym;
+    h->otiles_all = all;
+    h->otile_of_sym = of_sym;
+    h->otile_of_all = of_all;
+}
+
+void upload_tiles(dfjk_handle* h) {
+    const auto& sym = h->otiles_sym;
+    const auto& all = h->otiles_all;
+    const auto& of_sym = h->otile_of_sym;
+    const auto& of_all = h->otile_of_all;
+    hcheck(hipMemcpy(h->d_tidx.get<int>(h->tidx.size()), h->tidx.data(), h->tidx.size() * sizeof(int),
+                     hipMemcpyHostToDevice),
+           "upload tidx");
+    hcheck(hipMemcpy(h->d_tiles.get<int2>(h->tiles.size()), h->tiles.data(), h->tiles.size() * sizeof(int2),
+                     hipMemcpyHostToDevice),
+           "upload tiles");
+    hcheck(hipMemcpy(h->d_otiles_sym.get<int2>(sym.size()), sym.data(), sym.size() * sizeof(int2),
+                     hipMemcpyHostToDevice),
+           "upload");
+    hcheck(hipMemcpy(h->d_otiles_all.get<int2>(all.size()), all.data(), all.size() * sizeof(int2),
+                     hipMemcpyHostToDevice),
+           "upload");
+    hcheck(hipMemcpy(h->d_otile_of_sym.get<int>(of_sym.size()), of_sym.data(), of_sym.size() * sizeof(int),
+                     hipMemcpyHostToDevice),
+           "upload");
+    hcheck(hipMemcpy(h->d_otile_of_all.get<int>(of_all.size()), of_all.data(), of_all.size() * sizeof(int),
+                     hipMemcpyHostToDevice),
+           "upload");
+}
+
+size_t b_doubles(const dfjk_handle* h) { return static_cast<size_t>(h->naux) * h->ntp * TILE2; }
+
+// Schwarz screening exactly as dfhelper.cc:299-420: tolerance = cutoff^2 / max (mn|mn).
+void prepare_sparsity(dfjk_handle* h) {
+    const int n = h->nbf;
+    const int ns = static_cast<int>(h->prim.shells.size());
+    std::vector<double> fun_max(static_cast<size_t>(n) * n), shell_max(static_cast<size_t>(ns) * ns);
+    const double max_val = schwarz_diagonal(h->prim, fun_max.data(), shell_max.data(), h->nthreads);
+    const double tol = h->cutoff * h->cutoff / max_val;
+    h->fun_mask.assign(static_cast<size_t>(n) * n, 0);
+    h->shell_mask.assign(static_cast<size_t>(ns) * ns, 0);
+    h->nsig = 0;
+    for (size_t i = 0; i < fun_max.size(); ++i)
+        if (fun_max[i] >= tol) {
+            h->fun_mask[i] = 1;
+            ++h->nsig;
+        }
+    for (size_t i = 0; i < shell_max.size(); ++i) h->shell_mask[i] = shell_max[i] >= tol;
+}
+
+// Metric factor on the host (FP64): returns M with B = M (P|mn).
+std::vector<double> metric_factor(dfjk_handle* h) {
+    const int na = h->naux;
+    std::vector<double> Jm(static_cast<size_t>(na) * na);
+    metric(h->aux, Jm.data(), h->nthreads);
+    std::vector<double> M(static_cast<size_t>(na) * na);
+    if (h->fitting == DFJK_FIT_CHOLESKY) {
+        const int bad = cholesky_lower(Jm.data(), na, h->nthreads);
+        if (bad >= 0)
+            fail(DFJK_ERR_ARG, "metric Cholesky failed at pivot " + std::to_string(bad) +
+                                   " (metric not positive definite; use DFJK_FIT_EIGEN)");
+        lower_inverse(Jm.data(), na, M.data(), h->nthreads);
+    } else {
+        fail(DFJK_ERR_ARG, "DFJK_FIT_EIGEN is not available in this build");
+    }
+    return M;
+}
+
+// Upload the host A = (P|mn) (tiled layout) and fit B = M A on the device, in column chunks.
+void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vector<double>& A) {
+    const int na = h->naux;
+    const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
+    double* dM = h->ws_G.get<double>(static_cast<size_t>(na) * na);
+    hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
+    double* dBp = h->dB.get<double>(b_doubles(h));
+    // chunk of columns so that the staging buffer stays <= ~1 GiB
+    size_t cw = std::max<size_t>(TILE2, (size_t(1) << 27) / std::max(1, na));
+    cw = std::min(ncolB, cw / TILE2 * TILE2);
+    double* dA = h->ws_XL.get<double>(static_cast<size_t>(na) * cw);
+    for (size_t c0 = 0; c0 < ncolB; c0 += cw) {
+        const size_t w = std::min(cw, ncolB - c0);
+        hcheck(hipMemcpy2DAsync(dA, w * sizeof(double), A.data() + c0, ncolB * sizeof(double), w * sizeof(double), na,
+                                hipMemcpyHostToDevice, h->stream),
+               "upload (P|mn)");
+        launch_gemm_nn_f64(dM, na, dA, static_cast<int>(w), dBp + c0, static_cast<int>(ncolB), na,
+                           static_cast<int>(w), na, 1.0, 0.0, h->stream);
+        hcheck(hipStreamSynchronize(h->stream), "fit");
+    }
+    h->ws_XL.release();
+    h->ws_G.release();
+}
+
+void initialize_basis(dfjk_handle* h) {
+    if (h->tiles.empty()) {
+        prepare_sparsity(h);
+        build_tiles(h);
+    }
+    upload_tiles(h);
+    std::vector<double> M = metric_factor(h);
+    // (P|mn) directly in the tiled layout: dst map per function pair
+    const int n = h->nbf;
+    std::vector<int64_t> dst(static_cast<size_t>(n) * n, -1);
+    for (int m = 0; m < n; ++m)
+        for (int q = 0; q < n; ++q) {
+            if (!h->fun_mask[static_cast<size_t>(m) * n + q]) continue;
+            const int I = m / TILE, J = q / TILE;
+            if (I > J) continue;  // lower block-triangle is not stored
+            const int t = h->tidx[I * h->nT + J];
+            dst[static_cast<size_t>(m) * n + q] = static_cast<int64_t>(t) * TILE2 + (m % TILE) * TILE + q % TILE;
+        }
+    const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
+    std::vector<double> A(static_cast<size_t>(h->naux) * ncolB, 0.0);
+    three_index(h->prim, h->aux, dst.data(), static_cast<int64_t>(ncolB), h->shell_mask.data(), A.data(),
+                h->nthreads);
+    fit_on_device(h, M, A);
+}
+
+// ---------------------------------------------------------------- compute
+struct PanelShape {
+    int nb, npanel, ldc;
+};
+PanelShape panel_shape(int ncol) {
+    const int nbt = std::max(1, (ncol + 15) / 16);
+    const int npanel = (nbt + 19) / 20;
+    const int nb = (nbt + npanel - 1) / npanel;
+    return {nb, npanel, npanel * 16 * nb};
+}
+
+int choose_nsplit(int ntile, int nbatch) {
+    const int target = 2048;
+    int ns = std::max(1, (target + ntile - 1) / ntile);
+    return std::min(ns, std::max(1, nbatch));
+}
+
+// One density pair: Cl/Cr are device [nbf][ncol] row-major (ld = ncol).
+void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol, double* dD, double* dJ, double* dK,
+                  int accumulate) {
+    const TileGeom g = h->geom();
+    const hipStream_t s = h->stream;
+    const bool sym = (dCr == nullptr);
+    const PanelShape ps = panel_shape(ncol);
+    const int q0 = h->aux_q0, q1 = h->aux_q1;
+    const int nq_total = q1 - q0;
+    EventTimer& tm = h->timer;
+
+    double* cl = h->ws_cl.get<double>(static_cast<size_t>(h->nbfp) * ps.ldc);
+    launch_pack_c(dCl, ncol, h->nbf, ncol, 1.0, cl, h->nbfp, ps.ldc, s);
+    double* cr = cl;
+    if (!sym) {
+        cr = h->ws_cr.get<double>(static_cast<size_t>(h->nbfp) * ps.ldc);
+        launch_pack_c(dCr, ncol, h->nbf, ncol, 1.0, cr, h->nbfp, ps.ldc, s);
+    }
+
+    // D = Cl Cr^T (jk.cc:314-354)
+    const bool need_D = (dD != nullptr) || (h->do_J && dJ != nullptr);
+    double* Dm = dD;
+    if (need_D) {
+        hipEvent_t e0 = tm.begin();
+        if (!Dm) Dm = h->ws_D.get<double>(static_cast<size_t>(h->nbf) * h->nbf);
+        double* P = h->ws_Pd.get<double>(static_cast<size_t>(h->ntile_all) * 16384);
+        launch_gemm_nt_f64(cl, 0, ps.ldc, cr, 0, ps.ldc, h->nbf, h->nbf, ps.ldc, 1, 1, h->d_otiles_all.as<int2>(),
+                           h->ntile_all, P, 0, s);
+        launch_tile_reduce(P, 1, h->ntile_all, h->d_otile_of_all.as<int>(), h->nB, 0, h->nbf, h->nbf, Dm, h->nbf, 1.0,
+                           0, s);
+        tm.end(T_OTHER, e0);
+    }
+
+    // J (dfhelper.cc:3162-3285): d_Q = <B_Q, D>, J = sum_Q B_Q d_Q
+    if (h->do_J && dJ) {
+        hipEvent_t e0 = tm.begin();
+        double* W = h->ws_W.get<double>(static_cast<size_t>(h->ntp) * TILE2);
+        double* dpart = h->ws_dpart.get<double>(static_cast<size_t>(h->ntp) * h->naux);
+        double* dvec = h->ws_d.get<double>(h->naux);
+        launch_pack_w(g, Dm, h->nbf, W, s);
+        hcheck(hipMemsetAsync(dpart, 0, sizeof(double) * static_cast<size_t>(h->ntp) * h->naux, s), "memset");
+        launch_j1(g, h->dB.as<double>(), W, h->naux, q0, q1, dpart, s);
+        launch_j1_reduce(g, dpart, h->naux, dvec, s);
+        // chunk count: enough workgroups to fill the device
+        const int want_wg = 4096;
+        int nchunk = std::max(1, std::min(nq_total, (want_wg + h->ntp - 1) / h->ntp));
+        const int qchunk = (h->naux + nchunk - 1) / nchunk;
+        nchunk = (h->naux + qchunk - 1) / qchunk;
+        double* Jpart = h->ws_Jpart.get<double>(static_cast<size_t>(nchunk) * h->ntp * TILE2);
+        launch_j2(g, h->dB.as<double>(), dvec, h->naux, qchunk, Jpart, s);
+        launch_j_unpack(g, Jpart, nchunk, dJ, h->nbf, 1.0, accumulate, s);
+        tm.end(T_J, e0);
+    }
+
+    // K (dfhelper.cc:2162-2186, 3350-3377): X_Q = B_Q C, K = sum_Q X_Q X'_Q^T
+    if (h->do_K && dK) {
+        const size_t xq = static_cast<size_t>(h->nbfp) * ps.ldc;  // doubles per Q
+        size_t budget = size_t(4) << 30;                            // bytes of X workspace
+        if (h->memory_doubles) {
+            const size_t resident = (b_doubles(h) + static_cast<size_t>(h->naux) * h->naux) * sizeof(double);
+            const size_t mem = h->memory_doubles * sizeof(double);
+            budget = mem > resident ? std::min(budget, mem - resident) : 0;
+        }
+        const size_t per_q = xq * sizeof(double) * (sym ? 1 : 2);
+        int qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
+        qblk = std::min(qblk, 65535);
+        const int nblocks = (nq_total + qblk - 1) / qblk;
+        qblk = (nq_total + nblocks - 1) / nblocks;  // balance
+        double* XL = h->ws_XL.get<double>(xq * qblk);
+        double* XR = sym ? XL : h->ws_XR.get<double>(xq * qblk);
+        const int ntile = sym ? h->ntile_sym : h->ntile_all;
+        const int2* otiles = sym ? h->d_otiles_sym.as<int2>() : h->d_otiles_all.as<int2>();
+        const int nsplit = choose_nsplit(ntile, qblk);
+        double* P = h->ws_P.get<double>(static_cast<size_t>(nsplit) * ntile * 16384);
+        for (int blk = 0; blk < nblocks; ++blk) {
+            const int qa = q0 + blk * qblk;
+            const int nq = std::min(qblk, q1 - qa);
+            hipEvent_t e0 = tm.begin();
+            launch_ht_f64(g, h->dB.as<double>(), qa, nq, cl, ps.ldc, ps.nb, ps.npanel, XL, s);
+            if (!sym) launch_ht_f64(g, h->dB.as<double>(), qa, nq, cr, ps.ldc, ps.nb, ps.npanel, XR, s);
+            tm.end(T_HT, e0);
+            h->n_ht += sym ? 1 : 2;
+            hipEvent_t e1 = tm.begin();
+            const int nsp = std::min(nsplit, nq);
+            launch_gemm_nt_f64(XL, xq, ps.ldc, XR, xq, ps.ldc, h->nbf, h->nbf, ps.ldc, nq, nsp, otiles, ntile, P,
+                               blk > 0 ? 1 : 0, s);
+            if (nsp < nsplit && blk == 0)
+                hcheck(hipMemsetAsync(P + static_cast<size_t>(nsp) * ntile * 16384, 0,
+                                      sizeof(double) * static_cast<size_t>(nsplit - nsp) * ntile * 16384, s),
+                       "memset");
+            tm.end(T_KGEMM, e1);
+            h->n_kgemm += 1;
+        }
+        hipEvent_t e2 = tm.begin();
+        launch_tile_reduce(P, nsplit, ntile, sym ? h->d_otile_of_sym.as<int>() : h->d_otile_of_all.as<int>(), h->nB,
+                           sym ? 1 : 0, h->nbf, h->nbf, dK, h->nbf, 1.0, accumulate, s);
+        tm.end(T_OTHER, e2);
+    }
+}
+
+void finish_timers(dfjk_handle* h) {
+    if (!h->timer.on) return;
+    hcheck(hipStreamSynchronize(h->stream), "sync");
+    for (auto& r : h->timer.recs) {
+        float ms = 0.f;
+        hcheck(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+        h->t_ms[r.cls] += ms;
+        h->timer.pool.push_back(r.a);
+        h->timer.pool.push_back(r.b);
+    }
+    h->timer.recs.clear();
+}
+
+void reset_timers(dfjk_handle* h) {
+    for (double& t : h->t_ms) t = 0.0;
+    h->n_ht = h->n_kgemm = 0;
+    ensure_stream(h);
+}
+
+void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                  double* const* D, double* const* J, double* const* K, int accumulate, bool device) {
+    require_state(h, true);
+    if (n < 0 || (n > 0 && (!Cl || !ncol))) fail(DFJK_ERR_ARG, "JK: bad C_left arguments");
+    if (h->precision != DFJK_PREC_FP64) fail(DFJK_ERR_ARG, "requested precision is not available in this build");
+    reset_timers(h);
+    const size_t nn = static_cast<size_t>(h->nbf) * h->nbf;
+    for (int i = 0; i < n; ++i) {
+        if (ncol[i] < 0) fail(DFJK_ERR_ARG, "JK: negative column count");
+        if (!Cl[i] && ncol[i] > 0) fail(DFJK_ERR_ARG, "JK: null C_left matrix");
+        if (Cr && !Cr[i] && ncol[i] > 0) fail(DFJK_ERR_ARG, "JK: null C_right matrix");
+        double* dD = D ? D[i] : nullptr;
+        double* dJ = J ? J[i] : nullptr;
+        double* dK = K ? K[i] : nullptr;
+        if (ncol[i] == 0) {
+            // psi4: D = 0, J += 0, K skipped for nocc == 0 (dfhelper.cc:3355-3358)
+            if (device) {
+                if (dD) hcheck(hipMemsetAsync(dD, 0, nn * sizeof(double), h->stream), "memset");
+                if (dJ && !accumulate) hcheck(hipMemsetAsync(dJ, 0, nn * sizeof(double), h->stream), "memset");
+                if (dK && !accumulate) hcheck(hipMemsetAsync(dK, 0, nn * sizeof(double), h->stream), "memset");
+            } else {
+                if (dD) std::memset(dD, 0, nn * sizeof(double));
+                if (dJ && !accumulate) std::memset(dJ, 0, nn * sizeof(double));
+                if (dK && !accumulate) std::memset(dK, 0, nn * sizeof(double));
+            }
+            continue;
+        }
+        const size_t csz = static_cast<size_t>(h->nbf) * ncol[i];
+        const double* cl = Cl[i];
+        const double* cr = Cr ? Cr[i] : nullptr;
+        double *gD = dD, *gJ = dJ, *gK = dK;
+        if (!device) {
+            double* raw = h->ws_craw.get<double>(2 * csz);
+            hcheck(hipMemcpyAsync(raw, cl, csz * sizeof(double), hipMemcpyHostToDevice, h->stream), "upload C_left");
+            cl = raw;
+            if (cr) {
+                hcheck(hipMemcpyAsync(raw + csz, cr, csz * sizeof(double), hipMemcpyHostToDevice, h->stream),
+                       "upload C_right");
+                cr = raw + csz;
+            }
+            gD = dD ? h->ws_D.get<double>(nn) : nullptr;
+            gJ = dJ ? h->ws_J.get<double>(nn) : nullptr;
+            gK = dK ? h->ws_K.get<double>(nn) : nullptr;
+            if (accumulate) {
+                if (gJ) hcheck(hipMemcpyAsync(gJ, dJ, nn * sizeof(double), hipMemcpyHostToDevice, h->stream), "up");
+                if (gK) hcheck(hipMemcpyAsync(gK, dK, nn * sizeof(double), hipMemcpyHostToDevice, h->stream), "up");
+            }
+        }
+        compute_pair(h, cl, cr, ncol[i], gD, gJ, gK, accumulate);
+        if (!device) {
+            if (dD) hcheck(hipMemcpyAsync(dD, gD, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download D");
+            if (dJ && h->do_J)
+                hcheck(hipMemcpyAsync(dJ, gJ, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download J");
+            if (dK && h->do_K)
+                hcheck(hipMemcpyAsync(dK, gK, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download K");
+            hcheck(hipStreamSynchronize(h->stream), "compute");
+        }
+    }
+    finish_timers(h);
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+const char* dfjk_last_error(void) { return g_err.c_str(); }
+
+int dfjk_create(dfjk_handle** out, const p4a_basis* primary, const p4a_basis* aux) {
+    return guarded([&] {
+        if (!out || !primary || !aux) fail(DFJK_ERR_ARG, "dfjk_create: null argument");
+        auto h = std::make_unique<dfjk_handle>();
+        h->prim = basis_from_desc(primary);
+        h->aux = basis_from_desc(aux);
+        h->nbf = h->prim.nbf;
+        h->naux = h->aux.nbf;
+        h->aux_q1 = h->naux;
+        h->nthreads = std::max(1, static_cast<int>(std::thread::hardware_concurrency()));
+        *out = h.release();
+    });
+}
+
+int dfjk_create_tensor(dfjk_handle** out, int nbf, int naux) {
+    return guarded([&] {
+        if (!out || nbf <= 0 || naux <= 0) fail(DFJK_ERR_ARG, "dfjk_create_tensor: bad sizes");
+        auto h = std::make_unique<dfjk_handle>();
+        h->tensor_mode = true;
+        h->nbf = nbf;
+        h->naux = naux;
+        h->aux_q1 = naux;
+        h->nthreads = std::max(1, static_cast<int>(std::thread::hardware_concurrency()));
+        *out = h.release();
+    });
+}
+
+int dfjk_destroy(dfjk_handle* h) {
+    return guarded([&] {
+        if (!h) return;
+        if (h->own_stream) (void)hipStreamSynchronize(h->own_stream);
+        hipStream_t s = h->own_stream;
+        delete h;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int dfjk_set_memory(dfjk_handle* h, size_t doubles) {
+    return guarded([&] {
+        require_state(h, false);
+        h->memory_doubles = doubles;
+    });
+}
+int dfjk_set_cutoff(dfjk_handle* h, double cutoff) {
+    return guarded([&] {
+        require_state(h, false);
+        if (!(cutoff >= 0.0)) fail(DFJK_ERR_ARG, "cutoff must be >= 0");
+        h->cutoff = cutoff;
+    });
+}
+int dfjk_set_condition(dfjk_handle* h, double condition) {
+    return guarded([&] {
+        require_state(h, false);
+        h->condition = condition;
+    });
+}
+int dfjk_set_do_J(dfjk_handle* h, int v) {
+    return guarded([&] {
+        require_state(h, false);
+        h->do_J = v != 0;
+    });
+}
+int dfjk_set_do_K(dfjk_handle* h, int v) {
+    return guarded([&] {
+        require_state(h, false);
+        h->do_K = v != 0;
+    });
+}
+int dfjk_set_omp_nthread(dfjk_handle* h, int n) {
+    return guarded([&] {
+        require_state(h, false);
+        if (n < 1) fail(DFJK_ERR_ARG, "omp_nthread must be >= 1");
+        h->nthreads = n;
+    });
+}
+int dfjk_set_precision(dfjk_handle* h, int p) {
+    return guarded([&] {
+        require_state(h, false);
+        if (p < DFJK_PREC_FP64 || p > DFJK_PREC_BF16X3) fail(DFJK_ERR_ARG, "unknown precision");
+        h->precision = p;
+    });
+}
+int dfjk_set_fitting(dfjk_handle* h, int f) {
+    return guarded([&] {
+        require_state(h, false);
+        if (f != DFJK_FIT_CHOLESKY && f != DFJK_FIT_EIGEN) fail(DFJK_ERR_ARG, "unknown fitting");
+        if (h->initialized) fail(DFJK_ERR_STATE, "fitting must be chosen before initialize()");
+        h->fitting = f;
+    });
+}
+int dfjk_set_stream(dfjk_handle* h, void* stream) {
+    return guarded([&] {
+        require_state(h, false);
+        h->stream = stream ? static_cast<hipStream_t>(stream) : h->own_stream;
+        h->timer.s = h->stream;
+    });
+}
+int dfjk_set_profiling(dfjk_handle* h, int on) {
+    return guarded([&] {
+        require_state(h, false);
+        h->timer.on = on != 0;
+    });
+}
+int dfjk_set_aux_range(dfjk_handle* h, int q0, int q1) {
+    return guarded([&] {
+        require_state(h, false);
+        if (q0 < 0 || q1 > h->naux || q0 >= q1) fail(DFJK_ERR_ARG, "aux range out of bounds");
+        h->aux_q0 = q0;
+        h->aux_q1 = q1;
+    });
+}
+
+int dfjk_memory_estimate(dfjk_handle* h, size_t* doubles) {
+    return guarded([&] {
+        require_state(h, false);
+        if (!doubles) fail(DFJK_ERR_ARG, "null output");
+        size_t ntp;
+        if (h->initialized || !h->tiles.empty()) {
+            ntp = h->ntp;
+        } else if (h->tensor_mode) {
+            const int nT = (h->nbf + TILE - 1) / TILE;
+            ntp = static_cast<size_t>(nT) * (nT + 1) / 2;
+        } else {
+            prepare_sparsity(h);
+            build_tiles(h);
+            ntp = h->ntp;
+        }
+        const size_t nbfp = static_cast<size_t>(round_up(h->nbf, TILE));
+        size_t est = static_cast<size_t>(h->naux) * ntp * TILE2;   // resident B
+        est += static_cast<size_t>(h->naux) * h->naux;            // metric (fit only)
+        est += 4 * static_cast<size_t>(h->nbf) * h->nbf;          // D, J, K, staging
+        est += 2 * nbfp * 320;                                    // padded C panels
+        *doubles = est;
+    });
+}
+
+int dfjk_initialize(dfjk_handle* h) {
+    return guarded([&] {
+        require_state(h, false);
+        if (h->initialized) return;
+        ensure_stream(h);
+        if (h->tensor_mode) {
+            h->fun_mask.assign(static_cast<size_t>(h->nbf) * h->nbf, 1);
+            h->nsig = static_cast<int64_t>(h->nbf) * h->nbf;
+            build_tiles(h);
+            upload_tiles(h);
+            h->dB.get<double>(b_doubles(h));
+            hcheck(hipMemsetAsync(h->dB.p, 0, b_doubles(h) * sizeof(double), h->stream), "memset B");
+            hcheck(hipStreamSynchronize(h->stream), "init");
+        } else {
+            initialize_basis(h);
+        }
+        h->initialized = true;
+    });
+}
+
+int dfjk_set_B_dense(dfjk_handle* h, const double* B, int on_device) {
+    return guarded([&] {
+        require_state(h, true);
+        if (!B) fail(DFJK_ERR_ARG, "null B");
+        const int n = h->nbf;
+        const size_t nn = static_cast<size_t>(n) * n;
+        std::vector<double> host;
+        const double* src = B;
+        if (on_device) {
+            host.resize(static_cast<size_t>(h->naux) * nn);
+            hcheck(hipMemcpy(host.data(), B, host.size() * sizeof(double), hipMemcpyDeviceToHost), "download B");
+            src = host.data();
+        }
+        const size_t qs = static_cast<size_t>(h->ntp) * TILE2;
+        std::vector<double> packed(qs);
+        for (int Q = 0; Q < h->naux; ++Q) {
+            std::fill(packed.begin(), packed.end(), 0.0);
+            const double* BQ = src + Q * nn;
+            for (int t = 0; t < h->ntp; ++t) {
+                const int2 IJ = h->tiles[t];
+                for (int r = 0; r < TILE; ++r)
+                    for (int c = 0; c < TILE; ++c) {
+                        const int m = IJ.x * TILE + r, q = IJ.y * TILE + c;
+                        if (m < n && q < n && h->fun_mask[static_cast<size_t>(m) * n + q])
+                            packed[static_cast<size_t>(t) * TILE2 + r * TILE + c] = BQ[static_cast<size_t>(m) * n + q];
+                    }
+            }
+            hcheck(hipMemcpy(h->dB.as<double>() + Q * qs, packed.data(), qs * sizeof(double), hipMemcpyHostToDevice),
+                   "upload B");
+        }
+    });
+}
+
+int dfjk_fill_synthetic(dfjk_handle* h, uint64_t seed) {
+    return guarded([&] {
+        require_state(h, true);
+        std::vector<double> decay(h->nbf);
+        for (int k = 0; k < h->nbf; ++k) decay[k] = std::exp(-static_cast<double>(k) / 64.0);
+        double* dd = h->ws_d.get<double>(std::max(h->nbf, h->naux));
+        hcheck(hipMemcpy(dd, decay.data(), decay.size() * sizeof(double), hipMemcpyHostToDevice), "upload decay");
+        const TileGeom g = h->geom();
+        for (int q0 = 0; q0 < h->naux; q0 += 32768) {
+            const int q1 = std::min(h->naux, q0 + 32768);
+            launch_synth_b(g, dd, seed, q0, q1, h->dB.as<double>(), h->stream);
+        }
+        hcheck(hipStreamSynchronize(h->stream), "synthetic B");
+    });
+}
+
+int dfjk_get_B_dense(dfjk_handle* h, int q0, int q1, double* out) {
+    return guarded([&] {
+        require_state(h, true);
+        if (!out || q0 < 0 || q1 > h->naux || q0 >= q1) fail(DFJK_ERR_ARG, "bad range");
+        const int n = h->nbf;
+        const size_t nn = static_cast<size_t>(n) * n;
+        const size_t qs = static_cast<size_t>(h->ntp) * TILE2;
+        std::vector<double> packed(qs);
+        for (int Q = q0; Q < q1; ++Q) {
+            hcheck(hipMemcpy(packed.data(), h->dB.as<double>() + Q * qs, qs * sizeof(double), hipMemcpyDeviceToHost),
+                   "download B");
+            double* o = out + (Q - q0) * nn;
+            std::fill(o, o + nn, 0.0);
+            for (int t = 0; t < h->ntp; ++t) {
+                const int2 IJ = h->tiles[t];
+                for (int r = 0; r < TILE; ++r)
+                    for (int c = 0; c < TILE; ++c) {
+                        const int m = IJ.x * TILE + r, q = IJ.y * TILE + c;
+                        if (m >= n || q >= n) continue;
+                        const double v = packed[static_cast<size_t>(t) * TILE2 + r * TILE + c];
+                        o[static_cast<size_t>(m) * n + q] = v;
+                        o[static_cast<size_t>(q) * n + m] = v;
+                    }
+            }
+        }
+    });
+}
+
+int dfjk_compute(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                 double* const* D, double* const* J, double* const* K, int accumulate) {
+    return guarded([&] { compute_impl(h, n, Cl, Cr, ncol, D, J, K, accumulate, false); });
+}
+
+int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                        double* const* D, double* const* J, double* const* K, int accumulate) {
+    return guarded([&] { compute_impl(h, n, Cl, Cr, ncol, D, J, K, accumulate, true); });
+}
+
+int dfjk_energy_adjoint_device(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB) {
+    return guarded([&] {
+        (void)C;
+        (void)nocc;
+        (void)E;
+        (void)dEdC;
+        (void)dEdB;
+        require_state(h, true);
+        fail(DFJK_ERR_ARG, "energy adjoint is not available in this build");
+    });
+}
+
+int dfjk_get_timings(dfjk_handle* h, double* ms_j, double* ms_ht, double* ms_kgemm, double* ms_other, int* n_ht,
+                     int* n_kgemm) {
+    return guarded([&] {
+        require_state(h, false);
+        if (ms_j) *ms_j = h->t_ms[T_J];
+        if (ms_ht) *ms_ht = h->t_ms[T_HT];
+        if (ms_kgemm) *ms_kgemm = h->t_ms[T_KGEMM];
+        if (ms_other) *ms_other = h->t_ms[T_OTHER];
+        if (n_ht) *n_ht = h->n_ht;
+        if (n_kgemm) *n_kgemm = h->n_kgemm;
+    });
+}
+
+int dfjk_get_info(dfjk_handle* h, int* nbf, int* naux, int* ntile_pairs, int64_t* nsig_pairs) {
+    return guarded([&] {
+        require_state(h, false);
+        if (nbf) *nbf = h->nbf;
+        if (naux) *naux = h->naux;
+        if (ntile_pairs) *ntile_pairs = h->ntp;
+        if (nsig_pairs) *nsig_pairs = h->nsig;
+    });
+}
+
+int dfjk_finalize(dfjk_handle* h) {
+    return guarded([&] {
+        require_state(h, false);
+        if (h->initialized && h->stream) hcheck(hipStreamSynchronize(h->stream), "finalize");
+        DevBuf* ws[] = {&h->ws_craw, &h->ws_cl, &h->ws_cr, &h->ws_D, &h->ws_W, &h->ws_dpart, &h->ws_d, &h->ws_Jpart,
+                        &h->ws_XL,   &h->ws_XR, &h->ws_P,  &h->ws_Pd, &h->ws_J, &h->ws_K,     &h->ws_G};
+        for (DevBuf* b : ws) b->release();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,479 @@
+// DF-JK device kernels for gfx950 (CDNA4), FP64 path.
+//
+// Data layout in HBM (all row-major, FP64):
+//   B      [naux][ntp][TILE][TILE]  fitted three-index tensor, stored as the
+//          TILE x TILE blocks (I,J), I <= J, of each symmetric B_Q; diagonal
+//          blocks hold both triangles.  Blocks with no Schwarz-significant pair
+//          are not stored (tile-granular screening).
+//   C      [nbfp][ldc]              orbital coefficients, zero-padded.
+//   X      [nq][nbfp][ldc]          half-transformed X_Q = B_Q C for a Q block.
+//
+// MFMA lane layouts used here (verified on the device):
+//   v_mfma_f64_16x16x4f64: A[i][k] in lane i+16k, B[k][j] in lane j+16k,
+//                          D[i][j] in lane j+16*(i%4), register i/4.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace p4a {
+
+namespace {
+
+inline dim3 grid1(size_t n, int block) { return dim3(static_cast<unsigned>((n + block - 1) / block)); }
+
+// ------------------------------------------------------------------ J path
+__global__ void k_pack_w(const double* __restrict__ D, int ldD, int nbf, const int2* __restrict__ tiles,
+                         double* __restrict__ W) {
+    const int t = blockIdx.x;
+    const int2 IJ = tiles[t];
+    for (int e = threadIdx.x; e < TILE2; e += blockDim.x) {
+        const int r = e / TILE, c = e % TILE;
+        const int m = IJ.x * TILE + r, n = IJ.y * TILE + c;
+        double v = 0.0;
+        if (m < nbf && n < nbf) {
+            v = D[static_cast<size_t>(m) * ldD + n];
+            if (IJ.x != IJ.y) v += D[static_cast<size_t>(n) * ldD + m];
+        }
+        W[static_cast<size_t>(t) * TILE2 + e] = v;
+    }
+}
+
+// One wave per Q: the wave holds the W tile (16 doubles per lane) and reduces
+// <B_Q tile, W tile> with a butterfly.  grid = (ntp, ceil((q1-q0)/qblk)), 256 threads.
+__global__ __launch_bounds__(256) void k_j1(const double* __restrict__ B, const double* __restrict__ W, int ntp,
+                                            int naux, int q0, int q1, int qblk, double* __restrict__ dpart) {
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double* Wt = W + static_cast<size_t>(t) * TILE2;
+    double w[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) w[k] = Wt[lane + 64 * k];
+    const int qa = q0 + blockIdx.y * qblk;
+    const int qb = min(q1, qa + qblk);
+    for (int Q = qa + wave; Q < qb; Q += 4) {
+        const double* Bt = B + (static_cast<size_t>(Q) * ntp + t) * TILE2;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s = fma(Bt[lane + 64 * k], w[k], s);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) dpart[static_cast<size_t>(t) * naux + Q] = s;
+    }
+}
+
+__global__ void k_j1_reduce(const double* __restrict__ dpart, int ntp, int naux, double* __restrict__ d) {
+    const int Q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (Q >= naux) return;
+    double s = 0.0;
+    for (int t = 0; t < ntp; ++t) s += dpart[static_cast<size_t>(t) * naux + Q];
+    d[Q] = s;
+}
+
+// grid = (ntp, nchunk), 256 threads, 4 elements per thread.
+__global__ __launch_bounds__(256) void k_j2(const double* __restrict__ B, const double* __restrict__ d, int ntp,
+                                            int naux, int qchunk, double* __restrict__ Jpart) {
+    const int t = blockIdx.x, c = blockIdx.y;
+    const int qa = c * qchunk, qb = min(naux, qa + qchunk);
+    const int e0 = threadIdx.x;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll 2
+    for (int Q = qa; Q < qb; ++Q) {
+        const double dq = d[Q];
+        const double* Bt = B + (static_cast<size_t>(Q) * ntp + t) * TILE2 + e0;
+        a0 = fma(Bt[0], dq, a0);
+        a1 = fma(Bt[256], dq, a1);
+        a2 = fma(Bt[512], dq, a2);
+        a3 = fma(Bt[768], dq, a3);
+    }
+    double* o = Jpart + (static_cast<size_t>(c) * ntp + t) * TILE2 + e0;
+    o[0] = a0;
+    o[256] = a1;
+    o[512] = a2;
+    o[768] = a3;
+}
+
+__global__ void k_j_unpack(const double* __restrict__ Jpart, int nchunk, int ntp, const int* __restrict__ tidx,
+                           int nT, int nbf, double* __restrict__ J, int ldJ, double alpha, int accumulate) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (n >= nbf) return;
+    int I = m / TILE, Jt = n / TILE, r = m % TILE, c = n % TILE;
+    if (I > Jt) {
+        int tmp = I;
+        I = Jt;
+        Jt = tmp;
+        tmp = r;
+        r = c;
+        c = tmp;
+    }
+    const int t = tidx[I * nT + Jt];
+    double s = 0.0;
+    if (t >= 0)
+        for (int ch = 0; ch < nchunk; ++ch) s += Jpart[(static_cast<size_t>(ch) * ntp + t) * TILE2 + r * TILE + c];
+    double* o = J + static_cast<size_t>(m) * ldJ + n;
+    *o = alpha * s + (accumulate ? *o : 0.0);
+}
+
+// ------------------------------------------------------------------ K path
+__global__ void k_pack_c(const double* __restrict__ src, int ldsrc, int nbf, int ncol, double scale,
+                         double* __restrict__ dst, int nbfp, int ldc) {
+    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<size_t>(nbfp) * ldc) return;
+    const int m = static_cast<int>(idx / ldc), i = static_cast<int>(idx % ldc);
+    dst[idx] = (m < nbf && i < ncol) ? scale * src[static_cast<size_t>(m) * ldsrc + i] : 0.0;
+}
+
+// Half-transform, FP64 MFMA.  One workgroup = 64 rows (two row tiles) of X_Q
+// for one Q and one column panel of 16*NB columns; 4 waves, wave w owns rows
+// 16w..16w+15 and all NB column blocks.  K runs over column tiles J in steps
+// of 16; tiles below the block diagonal are read transposed from (J,I).
+// grid = (ceil(nT/2), nq, npanel).
+template <int NB>
+__global__ __launch_bounds__(256) void k_ht_f64(const double* __restrict__ B, const int* __restrict__ tidx, int nT,
+                                                int ntp, int q0, const double* __restrict__ C, int ldc,
+                                                double* __restrict__ X, int nbfp) {
+    constexpr int NPAN = 16 * NB;
+    constexpr int KS = 16;
+    __shared__ double As[64][KS + 1];
+    __shared__ double Cs[KS][NPAN + 16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ib = blockIdx.x;
+    const int q = blockIdx.y;
+    const int col0 = blockIdx.z * NPAN;
+    const double* BQ = B + static_cast<size_t>(q0 + q) * ntp * TILE2;
+    d4_t acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = d4_t{0.0, 0.0, 0.0, 0.0};
+    const int I0 = 2 * ib, I1 = 2 * ib + 1;
+    for (int J = 0; J < nT; ++J) {
+        const int t0 = tidx[min(I0, J) * nT + max(I0, J)];
+        const int t1 = (I1 < nT) ? tidx[min(I1, J) * nT + max(I1, J)] : -1;
+        if (t0 < 0 && t1 < 0) continue;  // uniform across the workgroup
+        for (int kh = 0; kh < TILE; kh += KS) {
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                const int e = tid + 256 * it;  // 0..1023
+                const int sub = e >> 9, f = e & 511;
+                const int I = sub ? I1 : I0;
+                const int t = sub ? t1 : t0;
+                int r, c;
+                double v = 0.0;
+                if (I <= J) {
+                    r = f >> 4;
+                    c = f & 15;
+                    if (t >= 0) v = BQ[static_cast<size_t>(t) * TILE2 + r * TILE + kh + c];
+                } else {
+                    c = f >> 5;
+                    r = f & 31;
+                    if (t >= 0) v = BQ[static_cast<size_t>(t) * TILE2 + (kh + c) * TILE + r];
+                }
+                As[sub * 32 + r][c] = v;
+            }
+            const double* Cj = C + static_cast<size_t>(J * TILE + kh) * ldc + col0;
+            for (int e = tid; e < KS * NPAN; e += 256) {
+                const int kk = e / NPAN, cc = e - kk * NPAN;
+                Cs[kk][cc] = Cj[static_cast<size_t>(kk) * ldc + cc];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < KS; ks += 4) {
+                const double a = As[wave * 16 + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const double bv = Cs[ks + (lane >> 4)][b * 16 + (lane & 15)];
+                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[b], 0, 0, 0);
+                }
+            }
+        }
+    }
+    double* Xq = X + static_cast<size_t>(q) * nbfp * ldc;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = ib * 64 + wave * 16 + 4 * r + (lane >> 4);
+            if (m < nbfp) Xq[static_cast<size_t>(m) * ldc + col0 + b * 16 + (lane & 15)] = acc[b][r];
+        }
+}
+
+// Batched NT product into 128x128 tiles, FP64 MFMA.  4 waves in a 2x2 grid,
+// each wave a 64x64 sub-tile (4x4 MFMA blocks).  grid = (ntile, nsplit).
+__global__ __launch_bounds__(256) void k_gemm_nt_f64(const double* __restrict__ A, size_t strideA, int lda,
+                                                     const double* __restrict__ Bm, size_t strideB, int ldb, int M,
+                                                     int N, int K, int nbatch, const int2* __restrict__ tiles,
+                                                     int ntile, double* __restrict__ P, int accumulate) {
+    __shared__ double As[128][17];
+    __shared__ double Bs[128][17];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int tile = blockIdx.x, split = blockIdx.y, nsplit = gridDim.y;
+    const int2 tb = tiles[tile];
+    const int m0 = tb.x * 128, n0 = tb.y * 128;
+    const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
+    const int b1 = static_cast<int>((static_cast<long long>(nbatch) * (split + 1)) / nsplit);
+    d4_t acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int b = b0; b < b1; ++b) {
+        const double* Ab = A + static_cast<size_t>(b) * strideA;
+        const double* Bb = Bm + static_cast<size_t>(b) * strideB;
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                const int e = tid + 256 * it;
+                const int r = e >> 4, kk = e & 15;
+                const bool kin = (k0 + kk) < K;
+                As[r][kk] = (kin && m0 + r < M) ? Ab[static_cast<size_t>(m0 + r) * lda + k0 + kk] : 0.0;
+                Bs[r][kk] = (kin && n0 + r < N) ? Bb[static_cast<size_t>(n0 + r) * ldb + k0 + kk] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < 16; ks += 4) {
+                double a[4], bv[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = As[wr * 64 + i * 16 + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bv[j] = Bs[wc * 64 + j * 16 + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i * 4 + j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i * 4 + j], 0, 0, 0);
+            }
+        }
+    }
+    double* Pt = P + (static_cast<size_t>(split) * ntile + tile) * 16384;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wr * 64 + i * 16 + 4 * r + (lane >> 4);
+                const int col = wc * 64 + j * 16 + (lane & 15);
+                double* o = Pt + row * 128 + col;
+                *o = acc[i * 4 + j][r] + (accumulate ? *o : 0.0);
+            }
+}
+
+__global__ void k_tile_reduce(const double* __restrict__ P, int nsplit, int ntile, const int* __restrict__ tile_of,
+                              int nB, int sym, int M, int N, double* __restrict__ out, int ldo, double alpha,
+                              int accumulate) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (n >= N) return;
+    int Ib = m >> 7, Jb = n >> 7, r = m & 127, c = n & 127;
+    if (sym && Ib > Jb) {
+        int tmp = Ib;
+        Ib = Jb;
+        Jb = tmp;
+        tmp = r;
+        r = c;
+        c = tmp;
+    }
+    const int t = tile_of[Ib * nB + Jb];
+    double s = 0.0;
+    if (t >= 0)
+        for (int sp = 0; sp < nsplit; ++sp) s += P[(static_cast<size_t>(sp) * ntile + t) * 16384 + r * 128 + c];
+    double* o = out + static_cast<size_t>(m) * ldo + n;
+    *o = alpha * s + (accumulate ? *o : 0.0);
+}
+
+// Row-major C = alpha*A*B + beta*C, 128x128 output tiles, FP64 MFMA.  grid = (ceil(N/128), ceil(M/128)).
+__global__ __launch_bounds__(256) void k_gemm_nn_f64(const double* __restrict__ A, int lda,
+                                                     const double* __restrict__ Bm, int ldb, double* __restrict__ Cm,
+                                                     int ldc, int M, int N, int K, double alpha, double beta) {
+    __shared__ double As[128][17];
+    __shared__ double Bs[16][132];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
+    d4_t acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int e = tid + 256 * it;
+            const int r = e >> 4, kk = e & 15;
+            As[r][kk] = (m0 + r < M && k0 + kk < K) ? A[static_cast<size_t>(m0 + r) * lda + k0 + kk] : 0.0;
+            const int kb = e >> 7, cc = e & 127;
+            Bs[kb][cc] = (k0 + kb < K && n0 + cc < N) ? Bm[static_cast<size_t>(k0 + kb) * ldb + n0 + cc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 16; ks += 4) {
+            double a[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[wr * 64 + i * 16 + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[ks + (lane >> 4)][wc * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i * 4 + j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i * 4 + j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 64 + i * 16 + 4 * r + (lane >> 4);
+                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+                if (row < M && col < N) {
+                    double* o = Cm + static_cast<size_t>(row) * ldc + col;
+                    *o = alpha * acc[i * 4 + j][r] + (beta != 0.0 ? beta * *o : 0.0);
+                }
+            }
+}
+
+// ------------------------------------------------------------------ synthetic B
+__device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c[0];
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c[2];
+        const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+        const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ inline double synth_normal(uint32_t m, uint32_t n, uint32_t Q, uint32_t k0, uint32_t k1) {
+    uint32_t c[4] = {m, n, Q, 0u};
+    philox4x32_10(c, k0, k1);
+    const uint64_t i1 = (static_cast<uint64_t>(c[0]) << 21) | (c[1] >> 11);
+    const uint64_t i2 = (static_cast<uint64_t>(c[2]) << 21) | (c[3] >> 11);
+    const double u1 = static_cast<double>(i1 + 1) * 0x1.0p-53;
+    const double u2 = static_cast<double>(i2) * 0x1.0p-53;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// grid = (ntp, q1-q0), 256 threads.
+__global__ void k_synth_b(const int2* __restrict__ tiles, int ntp, int nbf, const double* __restrict__ decay,
+                          uint32_t k0, uint32_t k1, int q0, double* __restrict__ B) {
+    const int t = blockIdx.x, Q = q0 + blockIdx.y;
+    const int2 IJ = tiles[t];
+    const double sQ = 1.0 / sqrt(static_cast<double>(Q) + 1.0);
+    double* Bt = B + (static_cast<size_t>(Q) * ntp + t) * TILE2;
+    for (int e = threadIdx.x; e < TILE2; e += blockDim.x) {
+        const int m = IJ.x * TILE + e / TILE, n = IJ.y * TILE + e % TILE;
+        double v = 0.0;
+        if (m < nbf && n < nbf) {
+            const double g = 0.5 * (synth_normal(m, n, Q, k0, k1) + synth_normal(n, m, Q, k0, k1));
+            v = sQ * decay[m > n ? m - n : n - m] * g;
+        }
+        Bt[e] = v;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+void launch_pack_w(const TileGeom& g, const double* D, int ldD, double* W, hipStream_t s) {
+    require(g.ntp > 0 && ldD >= g.nbf, "pack_w");
+    hipLaunchKernelGGL(k_pack_w, dim3(g.ntp), dim3(256), 0, s, D, ldD, g.nbf, g.d_tiles, W);
+}
+
+void launch_j1(const TileGeom& g, const double* B, const double* W, int naux, int q0, int q1, double* dpart,
+               hipStream_t s) {
+    require(q0 >= 0 && q1 <= naux && q0 < q1, "j1 q range");
+    const int qblk = 64;
+    dim3 grid(g.ntp, (q1 - q0 + qblk - 1) / qblk);
+    hipLaunchKernelGGL(k_j1, grid, dim3(256), 0, s, B, W, g.ntp, naux, q0, q1, qblk, dpart);
+}
+
+void launch_j1_reduce(const TileGeom& g, const double* dpart, int naux, double* d, hipStream_t s) {
+    hipLaunchKernelGGL(k_j1_reduce, grid1(naux, 256), dim3(256), 0, s, dpart, g.ntp, naux, d);
+}
+
+void launch_j2(const TileGeom& g, const double* B, const double* d, int naux, int qchunk, double* Jpart,
+               hipStream_t s) {
+    require(qchunk > 0, "j2 qchunk");
+    dim3 grid(g.ntp, (naux + qchunk - 1) / qchunk);
+    hipLaunchKernelGGL(k_j2, grid, dim3(256), 0, s, B, d, g.ntp, naux, qchunk, Jpart);
+}
+
+void launch_j_unpack(const TileGeom& g, const double* Jpart, int nchunk, double* J, int ldJ, double alpha,
+                     int accumulate, hipStream_t s) {
+    require(ldJ >= g.nbf, "j_unpack ld");
+    dim3 grid((g.nbf + 255) / 256, g.nbf);
+    hipLaunchKernelGGL(k_j_unpack, grid, dim3(256), 0, s, Jpart, nchunk, g.ntp, g.d_tidx, g.nT, g.nbf, J, ldJ, alpha,
+                       accumulate);
+}
+
+void launch_pack_c(const double* Csrc, int ldsrc, int nbf, int ncol, double scale, double* Cpad, int nbfp, int ldc,
+                   hipStream_t s) {
+    require(nbfp >= nbf && ldc >= ncol && ldsrc >= ncol, "pack_c");
+    const size_t n = static_cast<size_t>(nbfp) * ldc;
+    hipLaunchKernelGGL(k_pack_c, grid1(n, 256), dim3(256), 0, s, Csrc, ldsrc, nbf, ncol, scale, Cpad, nbfp, ldc);
+}
+
+template <int NB>
+static void ht_f64_nb(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int npanel,
+                      double* X, hipStream_t s) {
+    dim3 grid((g.nT + 1) / 2, nq, npanel);
+    hipLaunchKernelGGL(k_ht_f64<NB>, grid, dim3(256), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, C, ldc, X, g.nbfp);
+}
+
+void launch_ht_f64(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int nb, int npanel,
+                   double* X, hipStream_t s) {
+    require(nb >= 1 && nb <= 20 && ldc == npanel * 16 * nb && nq > 0 && g.nbfp == g.nT * TILE, "ht_f64");
+    switch (nb) {
+#define P4A_HT_CASE(n) \
+    case n:            \
+        ht_f64_nb<n>(g, B, q0, nq, C, ldc, npanel, X, s); \
+        break;
+        P4A_HT_CASE(1) P4A_HT_CASE(2) P4A_HT_CASE(3) P4A_HT_CASE(4) P4A_HT_CASE(5) P4A_HT_CASE(6) P4A_HT_CASE(7)
+        P4A_HT_CASE(8) P4A_HT_CASE(9) P4A_HT_CASE(10) P4A_HT_CASE(11) P4A_HT_CASE(12) P4A_HT_CASE(13)
+        P4A_HT_CASE(14) P4A_HT_CASE(15) P4A_HT_CASE(16) P4A_HT_CASE(17) P4A_HT_CASE(18) P4A_HT_CASE(19)
+        P4A_HT_CASE(20)
+#undef P4A_HT_CASE
+    }
+}
+
+void launch_gemm_nt_f64(const double* A, size_t strideA, int lda, const double* B, size_t strideB, int ldb, int M,
+                        int N, int K, int nbatch, int nsplit, const int2* d_tiles, int ntile, double* P, int accumulate,
+                        hipStream_t s) {
+    require(ntile > 0 && nsplit > 0 && nbatch >= nsplit && lda >= K && ldb >= K, "gemm_nt_f64");
+    dim3 grid(ntile, nsplit);
+    hipLaunchKernelGGL(k_gemm_nt_f64, grid, dim3(256), 0, s, A, strideA, lda, B, strideB, ldb, M, N, K, nbatch,
+                       d_tiles, ntile, P, accumulate);
+}
+
+void launch_tile_reduce(const double* P, int nsplit, int ntile, const int* d_tile_of, int nB, int sym, int M, int N,
+                        double* out, int ldo, double alpha, int accumulate, hipStream_t s) {
+    require(M <= nB * 128 && N <= nB * 128 && ldo >= N, "tile_reduce");
+    dim3 grid((N + 255) / 256, M);
+    hipLaunchKernelGGL(k_tile_reduce, grid, dim3(256), 0, s, P, nsplit, ntile, d_tile_of, nB, sym, M, N, out, ldo,
+                       alpha, accumulate);
+}
+
+void launch_gemm_nn_f64(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
+                        double alpha, double beta, hipStream_t s) {
+    require(lda >= K && ldb >= N && ldc >= N, "gemm_nn_f64");
+    dim3 grid((N + 127) / 128, (M + 127) / 128);
+    hipLaunchKernelGGL(k_gemm_nn_f64, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, alpha, beta);
+}
+
+void launch_synth_b(const TileGeom& g, const double* decay, uint64_t seed, int q0, int q1, double* B, hipStream_t s) {
+    require(q0 >= 0 && q1 > q0, "synth_b");
+    dim3 grid(g.ntp, q1 - q0);
+    hipLaunchKernelGGL(k_synth_b, grid, dim3(256), 0, s, g.d_tiles, g.ntp, g.nbf, decay,
+                       static_cast<uint32_t>(seed & 0xffffffffu), static_cast<uint32_t>(seed >> 32), q0, B);
+}
+
+}  // namespace p4a

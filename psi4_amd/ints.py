@@ -1,0 +1,104 @@
+"""ctypes bindings of the host integral engine (include/psi4amd_ints.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from ._libs import load_ints_lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(lib, rc):
+    if rc != 0:
+        raise RuntimeError(lib.p4a_ints_last_error().decode())
+
+
+def default_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def overlap(basis):
+    lib = load_ints_lib()
+    n = basis.nbf()
+    S = np.zeros((n, n))
+    cb, _keep = basis.as_c()
+    _check(lib, lib.p4a_ints_overlap(ctypes.byref(cb), _ptr(S)))
+    return S
+
+
+def kinetic(basis):
+    lib = load_ints_lib()
+    n = basis.nbf()
+    T = np.zeros((n, n))
+    cb, _keep = basis.as_c()
+    _check(lib, lib.p4a_ints_kinetic(ctypes.byref(cb), _ptr(T)))
+    return T
+
+
+def potential(basis, mol):
+    lib = load_ints_lib()
+    n = basis.nbf()
+    V = np.zeros((n, n))
+    cb, _keep = basis.as_c()
+    Z = np.ascontiguousarray(mol.Z, np.float64)
+    xyz = np.ascontiguousarray(mol.xyz, np.float64)
+    _check(lib, lib.p4a_ints_potential(ctypes.byref(cb), mol.natom, _ptr(Z), _ptr(xyz), _ptr(V)))
+    return V
+
+
+def metric(aux, nthreads=None):
+    lib = load_ints_lib()
+    n = aux.nbf()
+    J = np.zeros((n, n))
+    cb, _keep = aux.as_c()
+    _check(lib, lib.p4a_ints_metric(ctypes.byref(cb), _ptr(J), nthreads or default_threads()))
+    return J
+
+
+def three_index(prim, aux, nthreads=None):
+    """Dense (P|mn) as an array [naux, nbf, nbf]."""
+    lib = load_ints_lib()
+    n, na = prim.nbf(), aux.nbf()
+    out = np.zeros((na, n, n))
+    dst = np.arange(n * n, dtype=np.int64)
+    cp, _k1 = prim.as_c()
+    ca, _k2 = aux.as_c()
+    _check(lib, lib.p4a_ints_three_index(ctypes.byref(cp), ctypes.byref(ca), dst.ctypes.data_as(
+        ctypes.POINTER(ctypes.c_int64)), ctypes.c_int64(n * n), None, _ptr(out), nthreads or default_threads()))
+    return out
+
+
+def schwarz(prim, nthreads=None):
+    """(fun_max [nbf,nbf], shell_max [nshell,nshell], max_val) per dfhelper.cc:322-366."""
+    lib = load_ints_lib()
+    n, ns = prim.nbf(), prim.nshell
+    fm = np.zeros((n, n))
+    sm = np.zeros((ns, ns))
+    mx = ctypes.c_double(0)
+    cp, _k = prim.as_c()
+    _check(lib, lib.p4a_ints_schwarz(ctypes.byref(cp), _ptr(fm), _ptr(sm), ctypes.byref(mx),
+                                     nthreads or default_threads()))
+    return fm, sm, mx.value
+
+
+def four_index(basis, nthreads=None):
+    lib = load_ints_lib()
+    n = basis.nbf()
+    out = np.zeros((n, n, n, n))
+    cb, _k = basis.as_c()
+    _check(lib, lib.p4a_ints_four_index(ctypes.byref(cb), _ptr(out), nthreads or default_threads()))
+    return out
+
+
+def boys(nmax, T):
+    lib = load_ints_lib()
+    F = np.zeros(nmax + 1)
+    _check(lib, lib.p4a_ints_boys(nmax, ctypes.c_double(T), _ptr(F)))
+    return F
