@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the DF-JK path on one MI355X.
+
+Metric: J+K builds/s at nbf=1500, naux=6000, nocc=300 (mixed vs FP64), |dE| vs FP64.
+Workload: the seeded synthetic tensor-level case -- B_Qmn = s_Q exp(-|m-n|/64)
+(G_Qmn+G_Qnm)/2 generated on the device, C_occ = first 300 columns of the QR
+factor of a seeded N(0,1) 1500x1500 matrix.  One step = one symmetric J+K build
+(D = C C^T, J, K) with C already resident in HBM and J, K left in HBM.
+
+Printed: one JSON line (rank 0) with the headline precision's builds/s as
+``value``, the FP64 builds/s beside it, |dE| = |E_mixed - E_fp64| with
+E = tr(D(2J-K)), the roofline of the dominant kernel (HIP events on the
+library's stream over the timed region), and the CPU oracle timed on a
+bounded Q-slice of the same workload (J and K are additive over Q).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X dense peaks in algorithmic TFLOP/s: FP64 MFMA, FP32 MFMA, and the BF16 MFMA
+# peak divided by the three products of the 3-term split.
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="bf16x3", help="headline (mixed) precision: fp32|bf16x3|fp64")
+    ap.add_argument("--no-incremental", action="store_true", help="skip the incremental (dD) build measurement")
+    ap.add_argument("--cpu-sample", type=int, default=200, help="aux functions in the CPU oracle slice (0: skip)")
+    ap.add_argument("--nbf", type=int, default=None)
+    ap.add_argument("--naux", type=int, default=None)
+    ap.add_argument("--nocc", type=int, default=None)
+    ap.add_argument("--skip-fp64", action="store_true", help="do not time the FP64 reference build")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding over the aux index is a design "
+                                   "(see DESIGN.md), not a measured path", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from psi4_amd.core import JK, PREC_BF16X3, PREC_FP32, PREC_FP64
+    from psi4_amd.workloads import SYNTHETIC, jk_flops, synthetic_C
+
+    nbf = args.nbf or SYNTHETIC["nbf"]
+    naux = args.naux or SYNTHETIC["naux"]
+    nocc = args.nocc or SYNTHETIC["nocc"]
+    dev = torch.device("cuda:0")
+    prec_id = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+    headline = args.precision
+    if headline not in prec_id:
+        raise SystemExit(f"unknown precision {headline}")
+
+    log(f"synthetic workload nbf={nbf} naux={naux} nocc={nocc}: generating B on the device")
+    jk = JK.build_tensor(nbf, naux)
+    jk.set_print(0)
+    stream = torch.cuda.current_stream(dev)
+    jk.set_stream(stream.cuda_stream)
+    jk.initialize()
+    jk.fill_synthetic(SYNTHETIC["seed_B"])
+    C_h = synthetic_C(nbf, nocc, SYNTHETIC["seed_C"])
+    C = torch.from_numpy(C_h).to(dev)
+    D = torch.empty(nbf, nbf, dtype=torch.float64, device=dev)
+    J = torch.empty_like(D)
+    K = torch.empty_like(D)
+    torch.cuda.synchronize()
+
+    def build():
+        jk.compute_device([C.data_ptr()], [], [nocc], [D.data_ptr()], [J.data_ptr()], [K.data_ptr()], False)
+
+    def energy():
+        return float(torch.sum(D * (2.0 * J - K)).item())
+
+    results = {}
+    order = [headline] + [p for p in ("fp32", "fp64") if p != headline and not (p == "fp64" and args.skip_fp64)]
+    for p in order:
+        jk.set_precision(prec_id[p])
+        jk.set_profiling(True)
+        log(f"{p}: {args.warmup} warmup + {args.steps} timed builds")
+        for _ in range(args.warmup):
+            build()
+        torch.cuda.synchronize()
+        tsum = {"J": 0.0, "HT": 0.0, "KGEMM": 0.0, "other": 0.0, "n_ht": 0, "n_kgemm": 0}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            build()
+            t = jk.timings()
+            for k in tsum:
+                tsum[k] += t[k]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        results[p] = dict(s_per_build=dt / args.steps, E=energy(), timings=tsum)
+        log(f"{p}: {dt / args.steps * 1e3:.2f} ms/build, E={results[p]['E']:.12f}, kernels(ms)="
+            f"J {tsum['J'] / args.steps:.2f} HT {tsum['HT'] / args.steps:.2f} KGEMM {tsum['KGEMM'] / args.steps:.2f}")
+    jk.set_profiling(False)
+
+    # Incremental build (the way the mixed path is used inside SCF): K(dD), J(dD) with
+    # C_left=[C_new, C_old], C_right=[C_new, -C_old] added to FP64 J/K of D_old.
+    inc = None
+    if not args.no_incremental:
+        rng = np.random.default_rng(SYNTHETIC["seed_C"] + 1)
+        Cold_h = np.linalg.qr(C_h + 1e-4 * rng.standard_normal(C_h.shape))[0]
+        Cold = torch.from_numpy(np.ascontiguousarray(Cold_h)).to(dev)
+        Jold = torch.empty_like(D)
+        Kold = torch.empty_like(D)
+        jk.set_precision(PREC_FP64)
+        jk.compute_device([Cold.data_ptr()], [], [nocc], [], [Jold.data_ptr()], [Kold.data_ptr()], False)
+        Cl = torch.cat([C, Cold], dim=1).contiguous()
+        Cr = torch.cat([C, -Cold], dim=1).contiguous()
+        Ji = torch.empty_like(D)
+        Ki = torch.empty_like(D)
+        jk.set_precision(prec_id[headline])
+        log(f"incremental {headline}: {args.warmup} warmup + {args.steps} timed dD builds (ncol={2 * nocc})")
+        for _ in range(args.warmup):
+            Ji.copy_(Jold)
+            Ki.copy_(Kold)
+            jk.compute_device([Cl.data_ptr()], [Cr.data_ptr()], [2 * nocc], [], [Ji.data_ptr()], [Ki.data_ptr()], True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            Ji.copy_(Jold)
+            Ki.copy_(Kold)
+            jk.compute_device([Cl.data_ptr()], [Cr.data_ptr()], [2 * nocc], [], [Ji.data_ptr()], [Ki.data_ptr()], True)
+        torch.cuda.synchronize()
+        dt_inc = (time.perf_counter() - t0) / args.steps
+        E_inc = float(torch.sum(D * (2.0 * Ji - Ki)).item()) if "fp64" in results else None
+        dD = float((C @ C.T - Cold @ Cold.T).abs().max().item())
+        inc = {"builds_per_s": round(1.0 / dt_inc, 4), "ms_per_build": round(dt_inc * 1e3, 3),
+               "precision": headline, "max_abs_dD": dD,
+               "dE_vs_fp64": (abs(E_inc - results["fp64"]["E"]) if E_inc is not None else None)}
+        log(f"incremental {headline}: {dt_inc * 1e3:.2f} ms/build, |dD|max={dD:.2e}, dE={inc['dE_vs_fp64']}")
+        jk.set_precision(prec_id[headline])
+
+    # roofline of the dominant kernel class of the headline build
+    hr = results[headline]
+    tt = hr["timings"]
+    fl_j, fl_ht, fl_kc = jk_flops(nbf, naux, nocc, symmetric=True)
+    dom = "HT" if tt["HT"] >= tt["KGEMM"] else "KGEMM"
+    nlaunch = tt["n_ht"] if dom == "HT" else tt["n_kgemm"]
+    flops_per_launch = (fl_ht if dom == "HT" else fl_kc) * args.steps / max(1, nlaunch)
+    ms_per_launch = tt[dom] / max(1, nlaunch)
+    achieved = flops_per_launch / (ms_per_launch * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK[headline], "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK[headline], 4), "traffic": None,
+                "kernel": ({"fp64": "k_ht_mfma<double>", "fp32": "k_ht_mfma<float>", "bf16x3": "k_ht_bf3"}[headline]
+                           + " (half-transform X_Q = B_Q C)") if dom == "HT" else
+                          ({"fp64": "k_kgemm_mfma<double>", "fp32": "k_kgemm_mfma<float>", "bf16x3": "k_kgemm_bf3"}[headline]
+                           + " (K = sum_Q X_Q X_Q^T)"),
+                "launches_per_build": nlaunch / args.steps, "ms_per_launch": round(ms_per_launch, 4),
+                "algorithmic_flops_per_launch": flops_per_launch}
+    # J path: bandwidth view (two streaming passes over packed B)
+    ntp = jk.info()["ntile_pairs"]
+    j_bytes = 2 * naux * ntp * 1024 * 8
+    j_ms = tt["J"] / args.steps
+    j_roof = {"bound": "hbm", "achieved_GBs": round(j_bytes / (j_ms * 1e-3) / 1e9, 1), "peak_GBs": HBM_PEAK_GBS,
+              "bytes_per_build": j_bytes, "ms_per_build": round(j_ms, 4)}
+
+    # CPU oracle on a bounded Q-slice of the same workload
+    cpu = None
+    if args.cpu_sample > 0:
+        qs = min(args.cpu_sample, naux)
+        log(f"CPU oracle on a Q-slice of {qs} aux functions")
+        from oracle.dfhelper import compute_J_symm, compute_K, prepare_sparsity
+        Bq = jk.get_B_dense(0, qs)
+        fm = np.ones((nbf, nbf))
+        sp = prepare_sparsity(fm, np.ones((1, 1)), 1.0, 1.0, qs)
+        Ppq = np.ascontiguousarray(Bq.transpose(1, 0, 2)).reshape(-1)
+        del Bq
+        Dh = C_h @ C_h.T
+        Jc = np.zeros((nbf, nbf))
+        Kc = np.zeros((nbf, nbf))
+        t0 = time.perf_counter()
+        compute_J_symm(Ppq, sp, Dh, Jc)
+        compute_K(Ppq, sp, C_h, C_h, Kc, True)
+        tcpu = time.perf_counter() - t0
+        try:
+            from threadpoolctl import threadpool_info
+            cores = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
+        except Exception:
+            cores = os.cpu_count()
+        # GPU on the same slice (FP64) for a same-run parity number
+        jk.set_precision(PREC_FP64)
+        jk.set_aux_range(0, qs)
+        build()
+        torch.cuda.synchronize()
+        dj = float(np.abs(J.cpu().numpy() - Jc).max())
+        dk = float(np.abs(K.cpu().numpy() - Kc).max())
+        jk.set_aux_range(0, naux)
+        cpu = {"value": round(1.0 / (tcpu * naux / qs), 5), "unit": "builds/s", "cores": cores, "kind": "port",
+               "sample": f"oracle compute_J_symm+compute_K on Q-slice [0,{qs}) of {naux}, {tcpu:.2f} s, "
+                         f"extrapolated x{naux / qs:.1f} (J, K additive over Q); host {platform.processor() or 'x86_64'}",
+               "slice_maxabs_dJ": dj, "slice_maxabs_dK": dk}
+        log(f"CPU slice {tcpu:.2f}s -> {cpu['value']} builds/s; GPU-vs-CPU slice |dJ|={dj:.2e} |dK|={dk:.2e}")
+
+    s = hr["s_per_build"]
+    out = {
+        "metric": "J+K builds/s at nbf=1500,naux=6000,nocc=300 (mixed vs FP64); |dE| vs FP64",
+        "value": round(1.0 / s, 4),
+        "unit": "builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 (K), f64 (J, accumulation)",
+                  "bf16x3": "bf16x3 split, f32 accumulate (K); f64 (J, accumulation)"}[headline],
+        "data": "synthetic (device Philox4x32-10 B, seeded QR C_occ)",
+        "config": {"workload": f"synthetic tensor-level DF-JK nbf={nbf} naux={naux} nocc={nocc}", "nbf": nbf,
+                   "naux": naux, "nocc": nocc, "precision": headline, "parallelism": "single GPU"},
+        "fp64_builds_per_s": round(1.0 / results["fp64"]["s_per_build"], 4) if "fp64" in results else None,
+        "speedup_vs_fp64": (round(results["fp64"]["s_per_build"] / s, 3) if "fp64" in results else None),
+        "dE_vs_fp64": (abs(results[headline]["E"] - results["fp64"]["E"]) if "fp64" in results else None),
+        "E_fp64": results.get("fp64", {}).get("E"),
+        "by_precision": {p: {"builds_per_s": round(1.0 / r["s_per_build"], 4), "E": r["E"],
+                             "dE_vs_fp64": (abs(r["E"] - results["fp64"]["E"]) if "fp64" in results else None),
+                             "kernel_ms_per_build": {k: round(v / args.steps, 4) for k, v in r["timings"].items()
+                                                     if not k.startswith("n_")}}
+                         for p, r in results.items()},
+        "incremental": inc,
+        "kernel_ms_per_build": {k: round(v / args.steps, 4) for k, v in tt.items() if not k.startswith("n_")},
+        "roofline": roofline,
+        "roofline_J": j_roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,7 +50,7 @@ def build_ints(verbose=False):
 
 
 def build_dfjk(verbose=False):
-    srcs = [os.path.join(CSRC, "dfjk", f) for f in ("kernels.hip", "dfjk.hip")]
+    srcs = [os.path.join(CSRC, "dfjk", f) for f in ("kernels.hip", "kernels_k.hip", "dfjk.hip")]
     out = os.path.join(LIB, "libpsi4amd_dfjk.so")
     _run([os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, "-shared", f"-I{INC}", *srcs,
           f"-L{LIB}", "-lp4a_ints", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"], verbose)

@@ -21,6 +21,7 @@
 #include "../ints/ints.hpp"
 #include "../ints/hostlinalg.hpp"
 #include "kernels.hpp"
+#include "kernels_k.hpp"
 #include "psi4amd_dfjk.h"
 
 namespace p4a {
@@ -167,10 +168,10 @@ struct dfjk_handle {
     std::vector<int2> otiles_sym, otiles_all;
     std::vector<int> otile_of_sym, otile_of_all;
     int ntile_sym = 0, ntile_all = 0;
-    // resident tensor
-    DevBuf dB;
+    // resident tensor and its lower-precision copies
+    DevBuf dB, dB32, dBh, dBl;
     // workspace
-    DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G;
+    DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G, ws_sign, ws_c32, ws_cbf, ws_cmap;
     // timing
     EventTimer timer;
     double t_ms[4] = {0, 0, 0, 0};
@@ -362,118 +363,244 @@ void initialize_basis(dfjk_handle* h) {
 }
 
 // ---------------------------------------------------------------- compute
-struct PanelShape {
-    int nb, npanel, ldc;
-};
-PanelShape panel_shape(int ncol) {
-    const int nbt = std::max(1, (ncol + 15) / 16);
-    const int npanel = (nbt + 19) / 20;
-    const int nb = (nbt + npanel - 1) / npanel;
-    return {nb, npanel, npanel * 16 * nb};
-}
-
 int choose_nsplit(int ntile, int nbatch) {
     const int target = 2048;
     int ns = std::max(1, (target + ntile - 1) / ntile);
     return std::min(ns, std::max(1, nbatch));
 }
 
+// Lower-precision copies of B, made once per tensor on first use.
+const float* ensure_B32(dfjk_handle* h) {
+    if (!h->dB32.p) {
+        float* p = h->dB32.get<float>(b_doubles(h));
+        launch_to_f32(h->dB.as<double>(), p, b_doubles(h), h->stream);
+    }
+    return h->dB32.as<float>();
+}
+void ensure_Bbf(dfjk_handle* h) {
+    if (!h->dBh.p) {
+        uint16_t* hi = h->dBh.get<uint16_t>(b_doubles(h));
+        uint16_t* lo = h->dBl.get<uint16_t>(b_doubles(h));
+        launch_to_bf_split(h->dB.as<double>(), hi, lo, b_doubles(h), h->stream);
+    }
+}
+void invalidate_copies(dfjk_handle* h) {
+    h->dB32.release();
+    h->dBh.release();
+    h->dBl.release();
+}
+
 // One density pair: Cl/Cr are device [nbf][ncol] row-major (ld = ncol).
+// Cr == nullptr: lr_symmetric.  If every column of Cr is bitwise +/- the same column
+// of Cl (the incremental C_left=[C_new,C_old], C_right=[C_new,-C_old] form), K is
+// built as sum_Q X_Q diag(s) X_Q^T: the columns are regrouped (+ first, then -), one
+// half-transform is run and the upper block-triangle is contracted twice, with +1
+// over the + group and -1 over the - group.
 void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol, double* dD, double* dJ, double* dK,
                   int accumulate) {
     const TileGeom g = h->geom();
     const hipStream_t s = h->stream;
-    const bool sym = (dCr == nullptr);
-    const PanelShape ps = panel_shape(ncol);
     const int q0 = h->aux_q0, q1 = h->aux_q1;
     const int nq_total = q1 - q0;
+    const int nbf = h->nbf, nbfp = h->nbfp;
+    const int xrows = (h->nT + 3) / 4 * 128;
     EventTimer& tm = h->timer;
 
-    double* cl = h->ws_cl.get<double>(static_cast<size_t>(h->nbfp) * ps.ldc);
-    launch_pack_c(dCl, ncol, h->nbf, ncol, 1.0, cl, h->nbfp, ps.ldc, s);
-    double* cr = cl;
+    // classify the pair and build the column map
+    bool sym = (dCr == nullptr);
+    std::vector<int> hs(ncol, 1);
     if (!sym) {
-        cr = h->ws_cr.get<double>(static_cast<size_t>(h->nbfp) * ps.ldc);
-        launch_pack_c(dCr, ncol, h->nbf, ncol, 1.0, cr, h->nbfp, ps.ldc, s);
+        int8_t* sg = h->ws_sign.get<int8_t>(ncol);
+        launch_column_signs(dCl, dCr, ncol, nbf, ncol, sg, s);
+        std::vector<int8_t> h8(ncol);
+        hcheck(hipMemcpyAsync(h8.data(), sg, ncol, hipMemcpyDeviceToHost, s), "signs");
+        hcheck(hipStreamSynchronize(s), "signs");
+        bool allpm = true;
+        for (int i = 0; i < ncol; ++i) {
+            hs[i] = h8[i];
+            allpm &= h8[i] != 0;
+        }
+        sym = allpm;
+    }
+    const bool general = !sym;
+    std::vector<int> plus, minus;
+    for (int i = 0; i < ncol; ++i) (hs[i] >= 0 ? plus : minus).push_back(i);
+    const bool signed_sym = sym && !minus.empty();
+    int ncol_eff = ncol, kplus = 0;
+    std::vector<int> cmap;
+    if (signed_sym) {
+        kplus = round_up(static_cast<int>(plus.size()), 32);
+        ncol_eff = kplus + round_up(static_cast<int>(minus.size()), 32);
+    }
+    int nbw, npanel, ldc;
+    ht_panel_shape(ncol_eff, &nbw, &npanel, &ldc);
+    cmap.assign(ldc, -1);
+    if (signed_sym) {
+        for (size_t i = 0; i < plus.size(); ++i) cmap[i] = plus[i];
+        for (size_t i = 0; i < minus.size(); ++i) cmap[kplus + i] = minus[i];
+    } else {
+        for (int i = 0; i < ncol; ++i) cmap[i] = i;
+    }
+    int* dmap = h->ws_cmap.get<int>(ldc);
+    hcheck(hipMemcpy(dmap, cmap.data(), ldc * sizeof(int), hipMemcpyHostToDevice), "column map");
+
+    double* cl = h->ws_cl.get<double>(static_cast<size_t>(nbfp) * ldc);
+    launch_pack_cmap_f64(dCl, ncol, nbf, dmap, cl, nbfp, ldc, s);
+    double* cr = cl;
+    if (dCr) {
+        cr = h->ws_cr.get<double>(static_cast<size_t>(nbfp) * ldc);
+        launch_pack_cmap_f64(dCr, ncol, nbf, dmap, cr, nbfp, ldc, s);
     }
 
-    // D = Cl Cr^T (jk.cc:314-354)
+    // D = Cl Cr^T (jk.cc:314-354), FP64
     const bool need_D = (dD != nullptr) || (h->do_J && dJ != nullptr);
     double* Dm = dD;
     if (need_D) {
         hipEvent_t e0 = tm.begin();
-        if (!Dm) Dm = h->ws_D.get<double>(static_cast<size_t>(h->nbf) * h->nbf);
+        if (!Dm) Dm = h->ws_D.get<double>(static_cast<size_t>(nbf) * nbf);
         double* P = h->ws_Pd.get<double>(static_cast<size_t>(h->ntile_all) * 16384);
-        launch_gemm_nt_f64(cl, 0, ps.ldc, cr, 0, ps.ldc, h->nbf, h->nbf, ps.ldc, 1, 1, h->d_otiles_all.as<int2>(),
-                           h->ntile_all, P, 0, s);
-        launch_tile_reduce(P, 1, h->ntile_all, h->d_otile_of_all.as<int>(), h->nB, 0, h->nbf, h->nbf, Dm, h->nbf, 1.0,
-                           0, s);
+        launch_gemm_nt_f64(cl, 0, ldc, cr, 0, ldc, nbf, nbf, ldc, 1, 1, h->d_otiles_all.as<int2>(), h->ntile_all, P, 0,
+                           s);
+        launch_tile_reduce(P, 1, h->ntile_all, h->d_otile_of_all.as<int>(), h->nB, 0, nbf, nbf, Dm, nbf, 1.0, 0, s);
         tm.end(T_OTHER, e0);
     }
 
-    // J (dfhelper.cc:3162-3285): d_Q = <B_Q, D>, J = sum_Q B_Q d_Q
+    // J (dfhelper.cc:3162-3285): d_Q = <B_Q, D>, J = sum_Q B_Q d_Q, FP64 at every precision
     if (h->do_J && dJ) {
         hipEvent_t e0 = tm.begin();
         double* W = h->ws_W.get<double>(static_cast<size_t>(h->ntp) * TILE2);
         double* dpart = h->ws_dpart.get<double>(static_cast<size_t>(h->ntp) * h->naux);
         double* dvec = h->ws_d.get<double>(h->naux);
-        launch_pack_w(g, Dm, h->nbf, W, s);
-        hcheck(hipMemsetAsync(dpart, 0, sizeof(double) * static_cast<size_t>(h->ntp) * h->naux, s), "memset");
+        launch_pack_w(g, Dm, nbf, W, s);
+        if (q0 > 0 || q1 < h->naux)
+            hcheck(hipMemsetAsync(dpart, 0, sizeof(double) * static_cast<size_t>(h->ntp) * h->naux, s), "memset");
         launch_j1(g, h->dB.as<double>(), W, h->naux, q0, q1, dpart, s);
         launch_j1_reduce(g, dpart, h->naux, dvec, s);
-        // chunk count: enough workgroups to fill the device
         const int want_wg = 4096;
         int nchunk = std::max(1, std::min(nq_total, (want_wg + h->ntp - 1) / h->ntp));
         const int qchunk = (h->naux + nchunk - 1) / nchunk;
         nchunk = (h->naux + qchunk - 1) / qchunk;
         double* Jpart = h->ws_Jpart.get<double>(static_cast<size_t>(nchunk) * h->ntp * TILE2);
         launch_j2(g, h->dB.as<double>(), dvec, h->naux, qchunk, Jpart, s);
-        launch_j_unpack(g, Jpart, nchunk, dJ, h->nbf, 1.0, accumulate, s);
+        launch_j_unpack(g, Jpart, nchunk, dJ, nbf, 1.0, accumulate, s);
         tm.end(T_J, e0);
     }
 
     // K (dfhelper.cc:2162-2186, 3350-3377): X_Q = B_Q C, K = sum_Q X_Q X'_Q^T
     if (h->do_K && dK) {
-        const size_t xq = static_cast<size_t>(h->nbfp) * ps.ldc;  // doubles per Q
-        size_t budget = size_t(4) << 30;                            // bytes of X workspace
+        const int prec = h->precision;
+        const size_t xq = static_cast<size_t>(xrows) * ldc;  // elements per Q
+        const size_t esz = prec == DFJK_PREC_FP64 ? 8 : 4;    // FP64, FP32, or BF16 hi+lo
+        size_t budget = size_t(4) << 30;                      // bytes of X workspace
         if (h->memory_doubles) {
             const size_t resident = (b_doubles(h) + static_cast<size_t>(h->naux) * h->naux) * sizeof(double);
             const size_t mem = h->memory_doubles * sizeof(double);
             budget = mem > resident ? std::min(budget, mem - resident) : 0;
         }
-        const size_t per_q = xq * sizeof(double) * (sym ? 1 : 2);
+        const size_t per_q = xq * esz * (general ? 2 : 1);
         int qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
         qblk = std::min(qblk, 65535);
         const int nblocks = (nq_total + qblk - 1) / qblk;
-        qblk = (nq_total + nblocks - 1) / nblocks;  // balance
-        double* XL = h->ws_XL.get<double>(xq * qblk);
-        double* XR = sym ? XL : h->ws_XR.get<double>(xq * qblk);
+        qblk = (nq_total + nblocks - 1) / nblocks;
         const int ntile = sym ? h->ntile_sym : h->ntile_all;
         const int2* otiles = sym ? h->d_otiles_sym.as<int2>() : h->d_otiles_all.as<int2>();
         const int nsplit = choose_nsplit(ntile, qblk);
         double* P = h->ws_P.get<double>(static_cast<size_t>(nsplit) * ntile * 16384);
+        // contraction passes over k: (kbeg, kend, alpha)
+        struct Pass {
+            int kbeg, kend;
+            double alpha;
+        };
+        std::vector<Pass> passes;
+        if (signed_sym) {
+            if (kplus > 0) passes.push_back({0, kplus, 1.0});
+            passes.push_back({kplus, ncol_eff, -1.0});
+        } else {
+            passes.push_back({0, ldc, 1.0});
+        }
+
+        const float *B32 = nullptr, *c32l = nullptr, *c32r = nullptr;
+        uint16_t *cth_l = nullptr, *ctl_l = nullptr, *cth_r = nullptr, *ctl_r = nullptr;
+        const size_t csz = static_cast<size_t>(nbfp) * ldc;
+        if (prec == DFJK_PREC_FP32) {
+            B32 = ensure_B32(h);
+            float* a = h->ws_c32.get<float>(2 * csz);
+            launch_pack_cmap_f32(dCl, ncol, nbf, dmap, a, nbfp, ldc, s);
+            c32l = c32r = a;
+            if (general) {
+                launch_pack_cmap_f32(dCr, ncol, nbf, dmap, a + csz, nbfp, ldc, s);
+                c32r = a + csz;
+            }
+        } else if (prec == DFJK_PREC_BF16X3) {
+            ensure_Bbf(h);
+            uint16_t* a = h->ws_cbf.get<uint16_t>(4 * csz);
+            cth_l = a;
+            ctl_l = a + csz;
+            launch_pack_cT_bf(dCl, ncol, nbf, dmap, cth_l, ctl_l, nbfp, ldc, s);
+            cth_r = cth_l;
+            ctl_r = ctl_l;
+            if (general) {
+                cth_r = a + 2 * csz;
+                ctl_r = a + 3 * csz;
+                launch_pack_cT_bf(dCr, ncol, nbf, dmap, cth_r, ctl_r, nbfp, ldc, s);
+            }
+        }
+        char* XL = h->ws_XL.get<char>(xq * esz * qblk);
+        char* XR = general ? h->ws_XR.get<char>(xq * esz * qblk) : XL;
+
         for (int blk = 0; blk < nblocks; ++blk) {
             const int qa = q0 + blk * qblk;
             const int nq = std::min(qblk, q1 - qa);
-            hipEvent_t e0 = tm.begin();
-            launch_ht_f64(g, h->dB.as<double>(), qa, nq, cl, ps.ldc, ps.nb, ps.npanel, XL, s);
-            if (!sym) launch_ht_f64(g, h->dB.as<double>(), qa, nq, cr, ps.ldc, ps.nb, ps.npanel, XR, s);
-            tm.end(T_HT, e0);
-            h->n_ht += sym ? 1 : 2;
-            hipEvent_t e1 = tm.begin();
             const int nsp = std::min(nsplit, nq);
-            launch_gemm_nt_f64(XL, xq, ps.ldc, XR, xq, ps.ldc, h->nbf, h->nbf, ps.ldc, nq, nsp, otiles, ntile, P,
-                               blk > 0 ? 1 : 0, s);
+            hipEvent_t e0 = tm.begin();
+            if (prec == DFJK_PREC_FP64) {
+                launch_ht2_f64(g, h->dB.as<double>(), qa, nq, cl, ldc, nbw, npanel, reinterpret_cast<double*>(XL),
+                               xrows, s);
+                if (general)
+                    launch_ht2_f64(g, h->dB.as<double>(), qa, nq, cr, ldc, nbw, npanel, reinterpret_cast<double*>(XR),
+                                   xrows, s);
+            } else if (prec == DFJK_PREC_FP32) {
+                launch_ht2_f32(g, B32, qa, nq, c32l, ldc, nbw, npanel, reinterpret_cast<float*>(XL), xrows, s);
+                if (general)
+                    launch_ht2_f32(g, B32, qa, nq, c32r, ldc, nbw, npanel, reinterpret_cast<float*>(XR), xrows, s);
+            } else {
+                uint16_t* xl = reinterpret_cast<uint16_t*>(XL);
+                uint16_t* xr = reinterpret_cast<uint16_t*>(XR);
+                launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), qa, nq, cth_l, ctl_l, ldc, nbw, npanel,
+                              xl, xl + xq * qblk, xrows, s);
+                if (general)
+                    launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), qa, nq, cth_r, ctl_r, ldc, nbw,
+                                  npanel, xr, xr + xq * qblk, xrows, s);
+            }
+            tm.end(T_HT, e0);
+            h->n_ht += general ? 2 : 1;
+            hipEvent_t e1 = tm.begin();
+            for (size_t ip = 0; ip < passes.size(); ++ip) {
+                const Pass& pp = passes[ip];
+                const int acc = (blk > 0 || ip > 0) ? 1 : 0;
+                if (prec == DFJK_PREC_FP64) {
+                    launch_kgemm2_f64(reinterpret_cast<double*>(XL), reinterpret_cast<double*>(XR), xq, xrows, ldc,
+                                      pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha, acc, s);
+                } else if (prec == DFJK_PREC_FP32) {
+                    launch_kgemm2_f32(reinterpret_cast<float*>(XL), reinterpret_cast<float*>(XR), xq, xrows, ldc,
+                                      pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha, acc, s);
+                } else {
+                    uint16_t* xl = reinterpret_cast<uint16_t*>(XL);
+                    uint16_t* xr = reinterpret_cast<uint16_t*>(XR);
+                    launch_kgemm_bf3(xl, xl + xq * qblk, xr, xr + xq * qblk, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp,
+                                     otiles, ntile, P, pp.alpha, acc, s);
+                }
+                h->n_kgemm += 1;
+            }
             if (nsp < nsplit && blk == 0)
                 hcheck(hipMemsetAsync(P + static_cast<size_t>(nsp) * ntile * 16384, 0,
                                       sizeof(double) * static_cast<size_t>(nsplit - nsp) * ntile * 16384, s),
                        "memset");
             tm.end(T_KGEMM, e1);
-            h->n_kgemm += 1;
         }
         hipEvent_t e2 = tm.begin();
         launch_tile_reduce(P, nsplit, ntile, sym ? h->d_otile_of_sym.as<int>() : h->d_otile_of_all.as<int>(), h->nB,
-                           sym ? 1 : 0, h->nbf, h->nbf, dK, h->nbf, 1.0, accumulate, s);
+                           sym ? 1 : 0, nbf, nbf, dK, nbf, 1.0, accumulate, s);
         tm.end(T_OTHER, e2);
     }
 }
@@ -501,7 +628,6 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
                   double* const* D, double* const* J, double* const* K, int accumulate, bool device) {
     require_state(h, true);
     if (n < 0 || (n > 0 && (!Cl || !ncol))) fail(DFJK_ERR_ARG, "JK: bad C_left arguments");
-    if (h->precision != DFJK_PREC_FP64) fail(DFJK_ERR_ARG, "requested precision is not available in this build");
     reset_timers(h);
     const size_t nn = static_cast<size_t>(h->nbf) * h->nbf;
     for (int i = 0; i < n; ++i) {
@@ -734,6 +860,7 @@ int dfjk_set_B_dense(dfjk_handle* h, const double* B, int on_device) {
             hcheck(hipMemcpy(host.data(), B, host.size() * sizeof(double), hipMemcpyDeviceToHost), "download B");
             src = host.data();
         }
+        invalidate_copies(h);
         const size_t qs = static_cast<size_t>(h->ntp) * TILE2;
         std::vector<double> packed(qs);
         for (int Q = 0; Q < h->naux; ++Q) {
@@ -757,6 +884,7 @@ int dfjk_set_B_dense(dfjk_handle* h, const double* B, int on_device) {
 int dfjk_fill_synthetic(dfjk_handle* h, uint64_t seed) {
     return guarded([&] {
         require_state(h, true);
+        invalidate_copies(h);
         std::vector<double> decay(h->nbf);
         for (int k = 0; k < h->nbf; ++k) decay[k] = std::exp(-static_cast<double>(k) / 64.0);
         double* dd = h->ws_d.get<double>(std::max(h->nbf, h->naux));
@@ -848,7 +976,8 @@ int dfjk_finalize(dfjk_handle* h) {
         require_state(h, false);
         if (h->initialized && h->stream) hcheck(hipStreamSynchronize(h->stream), "finalize");
         DevBuf* ws[] = {&h->ws_craw, &h->ws_cl, &h->ws_cr, &h->ws_D, &h->ws_W, &h->ws_dpart, &h->ws_d, &h->ws_Jpart,
-                        &h->ws_XL,   &h->ws_XR, &h->ws_P,  &h->ws_Pd, &h->ws_J, &h->ws_K,     &h->ws_G};
+                        &h->ws_XL,   &h->ws_XR, &h->ws_P,  &h->ws_Pd, &h->ws_J, &h->ws_K,     &h->ws_G,
+                        &h->ws_sign, &h->ws_c32, &h->ws_cbf, &h->ws_cmap};
         for (DevBuf* b : ws) b->release();
     });
 }

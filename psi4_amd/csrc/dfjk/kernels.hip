@@ -6,7 +6,6 @@
 //          blocks hold both triangles.  Blocks with no Schwarz-significant pair
 //          are not stored (tile-granular screening).
 //   C      [nbfp][ldc]              orbital coefficients, zero-padded.
-//   X      [nq][nbfp][ldc]          half-transformed X_Q = B_Q C for a Q block.
 //
 // MFMA lane layouts used here (verified on the device):
 //   v_mfma_f64_16x16x4f64: A[i][k] in lane i+16k, B[k][j] in lane j+16k,
@@ -123,80 +122,6 @@ __global__ void k_pack_c(const double* __restrict__ src, int ldsrc, int nbf, int
     if (idx >= static_cast<size_t>(nbfp) * ldc) return;
     const int m = static_cast<int>(idx / ldc), i = static_cast<int>(idx % ldc);
     dst[idx] = (m < nbf && i < ncol) ? scale * src[static_cast<size_t>(m) * ldsrc + i] : 0.0;
-}
-
-// Half-transform, FP64 MFMA.  One workgroup = 64 rows (two row tiles) of X_Q
-// for one Q and one column panel of 16*NB columns; 4 waves, wave w owns rows
-// 16w..16w+15 and all NB column blocks.  K runs over column tiles J in steps
-// of 16; tiles below the block diagonal are read transposed from (J,I).
-// grid = (ceil(nT/2), nq, npanel).
-template <int NB>
-__global__ __launch_bounds__(256) void k_ht_f64(const double* __restrict__ B, const int* __restrict__ tidx, int nT,
-                                                int ntp, int q0, const double* __restrict__ C, int ldc,
-                                                double* __restrict__ X, int nbfp) {
-    constexpr int NPAN = 16 * NB;
-    constexpr int KS = 16;
-    __shared__ double As[64][KS + 1];
-    __shared__ double Cs[KS][NPAN + 16];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ib = blockIdx.x;
-    const int q = blockIdx.y;
-    const int col0 = blockIdx.z * NPAN;
-    const double* BQ = B + static_cast<size_t>(q0 + q) * ntp * TILE2;
-    d4_t acc[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = d4_t{0.0, 0.0, 0.0, 0.0};
-    const int I0 = 2 * ib, I1 = 2 * ib + 1;
-    for (int J = 0; J < nT; ++J) {
-        const int t0 = tidx[min(I0, J) * nT + max(I0, J)];
-        const int t1 = (I1 < nT) ? tidx[min(I1, J) * nT + max(I1, J)] : -1;
-        if (t0 < 0 && t1 < 0) continue;  // uniform across the workgroup
-        for (int kh = 0; kh < TILE; kh += KS) {
-            __syncthreads();
-#pragma unroll
-            for (int it = 0; it < 4; ++it) {
-                const int e = tid + 256 * it;  // 0..1023
-                const int sub = e >> 9, f = e & 511;
-                const int I = sub ? I1 : I0;
-                const int t = sub ? t1 : t0;
-                int r, c;
-                double v = 0.0;
-                if (I <= J) {
-                    r = f >> 4;
-                    c = f & 15;
-                    if (t >= 0) v = BQ[static_cast<size_t>(t) * TILE2 + r * TILE + kh + c];
-                } else {
-                    c = f >> 5;
-                    r = f & 31;
-                    if (t >= 0) v = BQ[static_cast<size_t>(t) * TILE2 + (kh + c) * TILE + r];
-                }
-                As[sub * 32 + r][c] = v;
-            }
-            const double* Cj = C + static_cast<size_t>(J * TILE + kh) * ldc + col0;
-            for (int e = tid; e < KS * NPAN; e += 256) {
-                const int kk = e / NPAN, cc = e - kk * NPAN;
-                Cs[kk][cc] = Cj[static_cast<size_t>(kk) * ldc + cc];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int ks = 0; ks < KS; ks += 4) {
-                const double a = As[wave * 16 + (lane & 15)][ks + (lane >> 4)];
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    const double bv = Cs[ks + (lane >> 4)][b * 16 + (lane & 15)];
-                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[b], 0, 0, 0);
-                }
-            }
-        }
-    }
-    double* Xq = X + static_cast<size_t>(q) * nbfp * ldc;
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int m = ib * 64 + wave * 16 + 4 * r + (lane >> 4);
-            if (m < nbfp) Xq[static_cast<size_t>(m) * ldc + col0 + b * 16 + (lane & 15)] = acc[b][r];
-        }
 }
 
 // Batched NT product into 128x128 tiles, FP64 MFMA.  4 waves in a 2x2 grid,
@@ -420,29 +345,6 @@ void launch_pack_c(const double* Csrc, int ldsrc, int nbf, int ncol, double scal
     require(nbfp >= nbf && ldc >= ncol && ldsrc >= ncol, "pack_c");
     const size_t n = static_cast<size_t>(nbfp) * ldc;
     hipLaunchKernelGGL(k_pack_c, grid1(n, 256), dim3(256), 0, s, Csrc, ldsrc, nbf, ncol, scale, Cpad, nbfp, ldc);
-}
-
-template <int NB>
-static void ht_f64_nb(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int npanel,
-                      double* X, hipStream_t s) {
-    dim3 grid((g.nT + 1) / 2, nq, npanel);
-    hipLaunchKernelGGL(k_ht_f64<NB>, grid, dim3(256), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, C, ldc, X, g.nbfp);
-}
-
-void launch_ht_f64(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int nb, int npanel,
-                   double* X, hipStream_t s) {
-    require(nb >= 1 && nb <= 20 && ldc == npanel * 16 * nb && nq > 0 && g.nbfp == g.nT * TILE, "ht_f64");
-    switch (nb) {
-#define P4A_HT_CASE(n) \
-    case n:            \
-        ht_f64_nb<n>(g, B, q0, nq, C, ldc, npanel, X, s); \
-        break;
-        P4A_HT_CASE(1) P4A_HT_CASE(2) P4A_HT_CASE(3) P4A_HT_CASE(4) P4A_HT_CASE(5) P4A_HT_CASE(6) P4A_HT_CASE(7)
-        P4A_HT_CASE(8) P4A_HT_CASE(9) P4A_HT_CASE(10) P4A_HT_CASE(11) P4A_HT_CASE(12) P4A_HT_CASE(13)
-        P4A_HT_CASE(14) P4A_HT_CASE(15) P4A_HT_CASE(16) P4A_HT_CASE(17) P4A_HT_CASE(18) P4A_HT_CASE(19)
-        P4A_HT_CASE(20)
-#undef P4A_HT_CASE
-    }
 }
 
 void launch_gemm_nt_f64(const double* A, size_t strideA, int lda, const double* B, size_t strideB, int ldb, int M,

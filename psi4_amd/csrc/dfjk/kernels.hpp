@@ -39,11 +39,6 @@ void launch_j_unpack(const TileGeom& g, const double* Jpart, int nchunk, double*
 // Pack a dense row-major nbf x ncol matrix (any leading dim) into [nbfp][ldc] zero-padded.
 void launch_pack_c(const double* Csrc, int ldsrc, int nbf, int ncol, double scale, double* Cpad, int nbfp, int ldc,
                    hipStream_t s);
-// Half-transform X[q][m][i] = sum_n B_{q0+q}[m][n] C[n][i] for q in [0,nq), FP64 MFMA.
-// C is [nbfp][ldc] with ldc = npanel * 16 * nb; X is [nq][nbfp][ldc].
-void launch_ht_f64(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int nb, int npanel,
-                   double* X, hipStream_t s);
-
 // Batched NT product accumulated into 128x128 output tiles:
 //   P[split][tile] (+)= sum_{b in split} A_b[m][:K] . B_b[n][:K]
 // A_b = A + b*strideA (row-major, lda), rows < M; likewise B (rows < N).

@@ -1,0 +1,765 @@
+// K-path kernels for gfx950: half-transform X_Q = B_Q C and the contraction
+// K = sum_Q X_Q S X_Q^T (S = diag of column signs, or a second operand X'),
+// in FP64, FP32 and 3-term BF16 split precision.
+//
+// All kernels are software pipelined: the next K-step's global loads are
+// issued into registers before the current step's MFMAs, then written to the
+// other half of a double-buffered LDS stage, with one barrier per step.
+//
+// MFMA lane layouts (checked on the device):
+//   f64 16x16x4 : A[i][k] lane i+16k; B[k][j] lane j+16k; D[i][j] lane j+16(i%4), reg i/4
+//   f32 16x16x4 : A, B as above;                          D[i][j] lane j+16(i/4), reg i%4
+//   bf16 16x16x32: A[i][8g+e] lane i+16g elem e; B[8g+e][j] lane j+16g elem e; D as f32
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+#include "kernels_k.hpp"
+
+namespace p4a {
+
+namespace {
+
+__device__ inline uint16_t f2bf(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// Row-tile lookup for a 128-row (four storage tiles) workgroup at column tile J.
+struct RowTiles {
+    int t[4];
+    bool any;
+    // select without dynamic indexing (keeps the array in registers)
+    __device__ int at(int s) const { return s == 0 ? t[0] : s == 1 ? t[1] : s == 2 ? t[2] : t[3]; }
+};
+__device__ inline RowTiles row_tiles(const int* __restrict__ tidx, int nT, int I0, int J) {
+    RowTiles rt;
+    rt.any = false;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int I = I0 + s;
+        rt.t[s] = (I < nT) ? tidx[min(I, J) * nT + max(I, J)] : -1;
+        rt.any |= rt.t[s] >= 0;
+    }
+    return rt;
+}
+// Element e of a 128 x KS chunk of A: row tile e / (32*KS), then (r, c) in the stored
+// tile's own orientation so that global reads are contiguous for both orientations.
+struct ACoord {
+    int sub, r, c;
+};
+template <int KS>
+__device__ inline ACoord a_coord(int e, int I0, int J) {
+    ACoord a;
+    a.sub = e / (32 * KS);
+    const int f = e % (32 * KS);
+    if (I0 + a.sub <= J) {
+        a.r = f / KS;
+        a.c = f % KS;
+    } else {
+        a.c = f / 32;
+        a.r = f % 32;
+    }
+    return a;
+}
+
+__device__ inline int next_present(const int* __restrict__ tidx, int nT, int I0, int J) {
+    while (J < nT && !row_tiles(tidx, nT, I0, J).any) ++J;
+    return J;
+}
+
+// =====================================================================================
+// Half-transform, FP64 / FP32 (16x16x4 MFMA).  512 threads = 8 waves (2 rows x 4 cols),
+// workgroup tile 128 rows x 64*NBW columns, wave tile 64 x 16*NBW, K-step 16.
+// T = double or float; B is the FP64 tensor (T=double) or its FP32 copy (T=float).
+// grid = (ceil(nT/4), nq, npanel)
+// =====================================================================================
+template <typename T, int NBW>
+__global__ __launch_bounds__(512) void k_ht_mfma(const T* __restrict__ B, const int* __restrict__ tidx, int nT,
+                                                 int ntp, int q0, const T* __restrict__ C, int ldc, T* __restrict__ X,
+                                                 int xrows) {
+    constexpr bool F64 = sizeof(T) == 8;
+    constexpr int PAN = 64 * NBW;
+    constexpr int KS = F64 ? 8 : 16;      // K-step: FP64 keeps the prefetch registers small
+    constexpr int AP = KS + 1;
+    constexpr int CP = PAN + 4;
+    constexpr int NA = 128 * KS / 512;    // A elements per thread per step
+    constexpr int NC = KS * PAN / 512;    // C elements per thread per step
+    __shared__ T As[2][128 * AP];
+    __shared__ T Cs[2][KS * CP];
+    typedef typename std::conditional<F64, d4_t, f4_t>::type acc_t;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 2, wn = wave & 3;
+    const int I0 = 4 * blockIdx.x;
+    const int q = blockIdx.y;
+    const int col0 = blockIdx.z * PAN;
+    const T* BQ = B + static_cast<size_t>(q0 + q) * ntp * TILE2;
+
+    acc_t acc[4][NBW];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    T ra[NA], rc[NC];
+    auto gload = [&](int J, int kh) {
+        const RowTiles rt = row_tiles(tidx, nT, I0, J);
+#pragma unroll
+        for (int it = 0; it < NA; ++it) {
+            const ACoord ac = a_coord<KS>(tid + 512 * it, I0, J);
+            const int t = rt.at(ac.sub);
+            T v = 0;
+            if (t >= 0) {
+                const T* tp = BQ + static_cast<size_t>(t) * TILE2;
+                v = (I0 + ac.sub <= J) ? tp[ac.r * TILE + kh + ac.c] : tp[(kh + ac.c) * TILE + ac.r];
+            }
+            ra[it] = v;
+        }
+        const T* Cj = C + static_cast<size_t>(J * TILE + kh) * ldc + col0;
+#pragma unroll
+        for (int it = 0; it < NC; ++it) {
+            const int e = tid + 512 * it;
+            const int kk = e / PAN, cc = e - kk * PAN;
+            rc[it] = Cj[static_cast<size_t>(kk) * ldc + cc];
+        }
+    };
+    auto lstore = [&](int buf, int J) {
+#pragma unroll
+        for (int it = 0; it < NA; ++it) {
+            const ACoord ac = a_coord<KS>(tid + 512 * it, I0, J);
+            As[buf][(ac.sub * 32 + ac.r) * AP + ac.c] = ra[it];
+        }
+#pragma unroll
+        for (int it = 0; it < NC; ++it) {
+            const int e = tid + 512 * it;
+            const int kk = e / PAN, cc = e - kk * PAN;
+            Cs[buf][kk * CP + cc] = rc[it];
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ks += 4) {
+            T a[4], b[NBW];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[buf][(wm * 64 + i * 16 + (lane & 15)) * AP + ks + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < NBW; ++j) b[j] = Cs[buf][(ks + (lane >> 4)) * CP + wn * NBW * 16 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < NBW; ++j) {
+                    if constexpr (F64)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                    else
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+                }
+        }
+    };
+
+    int J = next_present(tidx, nT, I0, 0);
+    int kh = 0;
+    if (J < nT) {
+        gload(J, kh);
+        lstore(0, J);
+        __syncthreads();
+        int buf = 0;
+        while (true) {
+            int nJ = J, nkh = kh + KS;
+            if (nkh == TILE) {
+                nkh = 0;
+                nJ = next_present(tidx, nT, I0, J + 1);
+            }
+            const bool more = nJ < nT;
+            if (more) gload(nJ, nkh);
+            compute(buf);
+            if (more) lstore(buf ^ 1, nJ);
+            __syncthreads();
+            if (!more) break;
+            buf ^= 1;
+            J = nJ;
+            kh = nkh;
+        }
+    }
+    // every row of the 128-row block is written (absent row tiles give zeros), so the
+    // X buffer needs no padding fill
+    T* Xq = X + static_cast<size_t>(q) * xrows * ldc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = F64 ? (4 * r + (lane >> 4)) : (4 * (lane >> 4) + r);
+                const int m = 128 * blockIdx.x + wm * 64 + i * 16 + rr;
+                Xq[static_cast<size_t>(m) * ldc + col0 + wn * NBW * 16 + j * 16 + (lane & 15)] = acc[i][j][r];
+            }
+}
+
+// =====================================================================================
+// Half-transform, 3-term BF16 split: X = Bh Ch + Bh Cl + Bl Ch with FP32 accumulation.
+// B as hi/lo BF16 copies, C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one
+// column tile (32).  Output X split again into hi/lo BF16 ([nq][nbfp][ldc]).
+// =====================================================================================
+template <int NBW>
+__global__ __launch_bounds__(512) void k_ht_bf3(const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl,
+                                                const int* __restrict__ tidx, int nT, int ntp, int q0,
+                                                const uint16_t* __restrict__ CTh, const uint16_t* __restrict__ CTl,
+                                                int nbfp, int ldc, uint16_t* __restrict__ Xh, uint16_t* __restrict__ Xl,
+                                                int xrows) {
+    constexpr int PAN = 64 * NBW;
+    constexpr int KP = 40;                // LDS pitch in bf16 (80 B rows, 16 B aligned)
+    constexpr int A_SZ = 128 * KP;
+    constexpr int C_SZ = PAN * KP;
+    constexpr int NCV = PAN * 4 / 512;    // 16-byte C vectors per thread per array (PAN*32/8/512)
+    constexpr int NCV_R = (PAN * 4) % 512;
+    __shared__ uint16_t Ls[2][2 * A_SZ + 2 * C_SZ];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 2, wn = wave & 3;
+    const int I0 = 4 * blockIdx.x;
+    const int q = blockIdx.y;
+    const int col0 = blockIdx.z * PAN;
+    const size_t qoff = static_cast<size_t>(q0 + q) * ntp * TILE2;
+
+    f4_t acc[4][NBW];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
+
+    // A: per row tile 32x32 bf16 = 128 vectors of 8; 4 row tiles -> 512 vectors per array.
+    static_assert(NCV + (NCV_R ? 1 : 0) <= 3, "C staging slots");
+    uint4 rah, ral;
+    uint4 c0h, c0l, c1h, c1l, c2h, c2l;
+    int at_flag;  // transposed?
+    auto gload = [&](int J) {
+        const RowTiles rt = row_tiles(tidx, nT, I0, J);
+        const int sub = tid >> 7, f = tid & 127;
+        const int t = rt.at(sub);
+        const int I = I0 + sub;
+        at_flag = (I > J);
+        rah = make_uint4(0, 0, 0, 0);
+        ral = rah;
+        if (t >= 0) {
+            const uint16_t* th = Bh + qoff + static_cast<size_t>(t) * TILE2;
+            const uint16_t* tl = Bl + qoff + static_cast<size_t>(t) * TILE2;
+            // vector f covers row (f>>2), cols 8*(f&3)..+7 of the stored tile
+            const int off = (f >> 2) * TILE + 8 * (f & 3);
+            rah = *reinterpret_cast<const uint4*>(th + off);
+            ral = *reinterpret_cast<const uint4*>(tl + off);
+        }
+        const size_t kb = static_cast<size_t>(J) * TILE;
+        auto cload = [&](int it, uint4& h, uint4& l) {
+            const int v = tid + 512 * it;
+            if (it < NCV || v < NCV_R + 512 * NCV) {
+                const int col = v >> 2, k8 = 8 * (v & 3);
+                const size_t off = static_cast<size_t>(col0 + col) * nbfp + kb + k8;
+                h = *reinterpret_cast<const uint4*>(CTh + off);
+                l = *reinterpret_cast<const uint4*>(CTl + off);
+            }
+        };
+        cload(0, c0h, c0l);
+        if (NCV >= 1) cload(1, c1h, c1l);
+        if (NCV >= 2) cload(2, c2h, c2l);
+    };
+    auto lstore = [&](int buf) {
+        uint16_t* Ah = Ls[buf];
+        uint16_t* Al = Ah + A_SZ;
+        uint16_t* Ch = Al + A_SZ;
+        uint16_t* Cl = Ch + C_SZ;
+        const int sub = tid >> 7, f = tid & 127;
+        const int r = f >> 2, c8 = 8 * (f & 3);
+        if (!at_flag) {
+            *reinterpret_cast<uint4*>(Ah + (sub * 32 + r) * KP + c8) = rah;
+            *reinterpret_cast<uint4*>(Al + (sub * 32 + r) * KP + c8) = ral;
+        } else {
+            // stored (J,I) tile: element [r][c8+e] is A[c8+e][r] of this row tile
+            const uint32_t hw[4] = {rah.x, rah.y, rah.z, rah.w};
+            const uint32_t lw[4] = {ral.x, ral.y, ral.z, ral.w};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int sh = 16 * (e & 1);
+                Ah[(sub * 32 + c8 + e) * KP + r] = static_cast<uint16_t>(hw[e >> 1] >> sh);
+                Al[(sub * 32 + c8 + e) * KP + r] = static_cast<uint16_t>(lw[e >> 1] >> sh);
+            }
+        }
+        auto cstore = [&](int it, const uint4& h, const uint4& l) {
+            const int v = tid + 512 * it;
+            if (it < NCV || v < NCV_R + 512 * NCV) {
+                const int col = v >> 2, k8 = 8 * (v & 3);
+                *reinterpret_cast<uint4*>(Ch + col * KP + k8) = h;
+                *reinterpret_cast<uint4*>(Cl + col * KP + k8) = l;
+            }
+        };
+        cstore(0, c0h, c0l);
+        if (NCV >= 1) cstore(1, c1h, c1l);
+        if (NCV >= 2) cstore(2, c2h, c2l);
+    };
+    auto compute = [&](int buf) {
+        const uint16_t* Ah = Ls[buf];
+        const uint16_t* Al = Ah + A_SZ;
+        const uint16_t* Ch = Al + A_SZ;
+        const uint16_t* Cl = Ch + C_SZ;
+        const int k8 = 8 * (lane >> 4);
+        bf8_t ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = wm * 64 + i * 16 + (lane & 15);
+            ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ah + row * KP + k8));
+            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + row * KP + k8));
+        }
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) {
+            const int col = wn * NBW * 16 + j * 16 + (lane & 15);
+            const bf8_t bh = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ch + col * KP + k8));
+            const bf8_t bl = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Cl + col * KP + k8));
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+            }
+        }
+    };
+
+    int J = next_present(tidx, nT, I0, 0);
+    if (J < nT) {
+        gload(J);
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        while (true) {
+            const int nJ = next_present(tidx, nT, I0, J + 1);
+            const bool more = nJ < nT;
+            if (more) gload(nJ);
+            compute(buf);
+            if (more) lstore(buf ^ 1);
+            __syncthreads();
+            if (!more) break;
+            buf ^= 1;
+            J = nJ;
+        }
+    }
+    const size_t xq = static_cast<size_t>(q) * xrows * ldc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = 128 * blockIdx.x + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+                const size_t o = xq + static_cast<size_t>(m) * ldc + col0 + wn * NBW * 16 + j * 16 + (lane & 15);
+                const float x = acc[i][j][r];
+                const uint16_t hi = f2bf(x);
+                Xh[o] = hi;
+                Xl[o] = f2bf(x - bf2f(hi));
+            }
+}
+
+// =====================================================================================
+// K contraction, FP64 / FP32 (16x16x4 MFMA): P[split][tile] (+)= sum_b A_b S B_b^T over
+// 128x128 output tiles, 4 waves (2x2) of 64x64, K-step 16, double-buffered LDS.
+// S = diag(sign) if sign != nullptr.  Accumulators are stored to FP64 partials.
+// =====================================================================================
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_mfma(const T* __restrict__ A, size_t strideA, const T* __restrict__ Bm,
+                                                    size_t strideB, int ld, int kbeg, int kend, int nbatch,
+                                                    const int2* __restrict__ tiles, int ntile, double* __restrict__ P,
+                                                    double alpha, int accumulate) {
+    constexpr int KS = 16, AP = KS + 1;
+    constexpr bool F64 = sizeof(T) == 8;
+    typedef typename std::conditional<F64, d4_t, f4_t>::type acc_t;
+    __shared__ T As[2][128 * AP];
+    __shared__ T Bs[2][128 * AP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int tile = blockIdx.x, split = blockIdx.y, nsplit = gridDim.y;
+    const int2 tb = tiles[tile];
+    const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
+    const int b1 = static_cast<int>((static_cast<long long>(nbatch) * (split + 1)) / nsplit);
+    const int nks = (kend - kbeg) / KS;
+    const int nsteps = (b1 - b0) * nks;
+    // thread's fixed (row, k) slots within a 128 x 16 chunk: rows r0 + 16*it, k = tid & 15
+    const int r0 = tid >> 4, kk = tid & 15;
+    const T* Ab = A + static_cast<size_t>(tb.x * 128 + r0) * ld + kbeg + kk;
+    const T* Bb = Bm + static_cast<size_t>(tb.y * 128 + r0) * ld + kbeg + kk;
+
+    acc_t acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = acc_t{0, 0, 0, 0};
+
+    T ra[8], rb[8];
+    int b = b0, ks0 = 0;
+    auto gload = [&]() {
+        const size_t oa = static_cast<size_t>(b) * strideA + ks0;
+        const size_t ob = static_cast<size_t>(b) * strideB + ks0;
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            ra[it] = Ab[oa + static_cast<size_t>(16 * it) * ld];
+            rb[it] = Bb[ob + static_cast<size_t>(16 * it) * ld];
+        }
+        ks0 += KS;
+        if (ks0 == nks * KS) {
+            ks0 = 0;
+            ++b;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            As[buf][(r0 + 16 * it) * AP + kk] = ra[it];
+            Bs[buf][(r0 + 16 * it) * AP + kk] = rb[it];
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ks += 4) {
+            T a[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[buf][(wr * 64 + i * 16 + (lane & 15)) * AP + ks + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[buf][(wc * 64 + j * 16 + (lane & 15)) * AP + ks + (lane >> 4)];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if constexpr (F64)
+                        acc[i * 4 + j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i * 4 + j], 0, 0, 0);
+                    else
+                        acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], bv[j], acc[i * 4 + j], 0, 0, 0);
+                }
+        }
+    };
+
+    if (nsteps > 0) {
+        gload();
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        for (int s = 0; s < nsteps; ++s) {
+            const bool more = s + 1 < nsteps;
+            if (more) gload();
+            compute(buf);
+            if (more) lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    double* Pt = P + (static_cast<size_t>(split) * ntile + tile) * 16384;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = F64 ? (4 * r + (lane >> 4)) : (4 * (lane >> 4) + r);
+                const int row = wr * 64 + i * 16 + rr;
+                const int col = wc * 64 + j * 16 + (lane & 15);
+                double* o = Pt + row * 128 + col;
+                *o = alpha * static_cast<double>(acc[i * 4 + j][r]) + (accumulate ? *o : 0.0);
+            }
+}
+
+// =====================================================================================
+// K contraction, 3-term BF16 split on hi/lo X: A S B^T ~ Ah Bh + Ah Bl + Al Bh.
+// 128x128 tiles, 4 waves of 64x64, K-step 32, double-buffered LDS.
+// =====================================================================================
+__global__ __launch_bounds__(256) void k_kgemm_bf3(const uint16_t* __restrict__ Ah_, const uint16_t* __restrict__ Al_,
+                                                   const uint16_t* __restrict__ Bh_, const uint16_t* __restrict__ Bl_,
+                                                   size_t stride, int ld, int kbeg, int kend, int nbatch,
+                                                   const int2* __restrict__ tiles, int ntile, double* __restrict__ P,
+                                                   double alpha, int accumulate) {
+    constexpr int KS = 32, KP = 40, SZ = 128 * KP;
+    __shared__ uint16_t Ls[2][4 * SZ];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int tile = blockIdx.x, split = blockIdx.y, nsplit = gridDim.y;
+    const int2 tb = tiles[tile];
+    const int m0 = tb.x * 128, n0 = tb.y * 128;
+    const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
+    const int b1 = static_cast<int>((static_cast<long long>(nbatch) * (split + 1)) / nsplit);
+    const int nks = (kend - kbeg) / KS;
+    const int nsteps = (b1 - b0) * nks;
+
+    f4_t acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = f4_t{0, 0, 0, 0};
+
+    // per step: 4 arrays x 128 rows x 32 k = 4 x 512 vectors of 8; 256 threads -> 2 vectors per array
+    // thread's two 8-element vectors per array: rows (tid>>2) and (tid>>2)+64, k8 = 8*(tid&3)
+    const int vrow = tid >> 2, vk8 = 8 * (tid & 3);
+    uint4 a0h, a0l, b0h, b0l, a1h, a1l, b1h, b1l;
+    auto gload = [&](int s) {
+        const int b = b0 + s / nks;
+        const int k0 = kbeg + (s % nks) * KS;
+        const size_t base = static_cast<size_t>(b) * stride + k0 + vk8;
+        const size_t oa0 = base + static_cast<size_t>(m0 + vrow) * ld;
+        const size_t oa1 = oa0 + static_cast<size_t>(64) * ld;
+        const size_t ob0 = base + static_cast<size_t>(n0 + vrow) * ld;
+        const size_t ob1 = ob0 + static_cast<size_t>(64) * ld;
+        a0h = *reinterpret_cast<const uint4*>(Ah_ + oa0);
+        a0l = *reinterpret_cast<const uint4*>(Al_ + oa0);
+        a1h = *reinterpret_cast<const uint4*>(Ah_ + oa1);
+        a1l = *reinterpret_cast<const uint4*>(Al_ + oa1);
+        b0h = *reinterpret_cast<const uint4*>(Bh_ + ob0);
+        b0l = *reinterpret_cast<const uint4*>(Bl_ + ob0);
+        b1h = *reinterpret_cast<const uint4*>(Bh_ + ob1);
+        b1l = *reinterpret_cast<const uint4*>(Bl_ + ob1);
+    };
+    auto lstore = [&](int buf) {
+        uint16_t* L = Ls[buf];
+        const int o0 = vrow * KP + vk8, o1 = (vrow + 64) * KP + vk8;
+        *reinterpret_cast<uint4*>(L + o0) = a0h;
+        *reinterpret_cast<uint4*>(L + SZ + o0) = a0l;
+        *reinterpret_cast<uint4*>(L + 2 * SZ + o0) = b0h;
+        *reinterpret_cast<uint4*>(L + 3 * SZ + o0) = b0l;
+        *reinterpret_cast<uint4*>(L + o1) = a1h;
+        *reinterpret_cast<uint4*>(L + SZ + o1) = a1l;
+        *reinterpret_cast<uint4*>(L + 2 * SZ + o1) = b1h;
+        *reinterpret_cast<uint4*>(L + 3 * SZ + o1) = b1l;
+    };
+    auto compute = [&](int buf) {
+        const uint16_t* L = Ls[buf];
+        const int k8 = 8 * (lane >> 4);
+        bf8_t ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = wr * 64 + i * 16 + (lane & 15);
+            ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + row * KP + k8));
+            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + SZ + row * KP + k8));
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = wc * 64 + j * 16 + (lane & 15);
+            const bf8_t bh = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + 2 * SZ + col * KP + k8));
+            const bf8_t bl = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + 3 * SZ + col * KP + k8));
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i * 4 + j], 0, 0, 0);
+                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i * 4 + j], 0, 0, 0);
+                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i * 4 + j], 0, 0, 0);
+            }
+        }
+    };
+
+    if (nsteps > 0) {
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        for (int s = 0; s < nsteps; ++s) {
+            const bool more = s + 1 < nsteps;
+            if (more) gload(s + 1);
+            compute(buf);
+            if (more) lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    double* Pt = P + (static_cast<size_t>(split) * ntile + tile) * 16384;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int row = wr * 64 + i * 16 + 4 * (lane >> 4) + rr;
+                const int col = wc * 64 + j * 16 + (lane & 15);
+                double* o = Pt + row * 128 + col;
+                *o = alpha * static_cast<double>(acc[i * 4 + j][rr]) + (accumulate ? *o : 0.0);
+            }
+}
+
+// ------------------------------------------------------------------ conversions
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = static_cast<float>(src[i]);
+}
+
+__global__ void k_to_bf_split(const double* __restrict__ src, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                              size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const float x = static_cast<float>(src[i]);
+        const uint16_t h = f2bf(x);
+        hi[i] = h;
+        lo[i] = f2bf(x - bf2f(h));
+    }
+}
+
+// C [nbf][ld] FP64 -> padded [nbfp][ldc] in T, column c taken from source column cmap[c] (< 0: zero)
+template <typename T>
+__global__ void k_pack_cmap(const double* __restrict__ src, int ldsrc, int nbf, const int* __restrict__ cmap,
+                            T* __restrict__ dst, int nbfp, int ldc) {
+    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<size_t>(nbfp) * ldc) return;
+    const int m = static_cast<int>(idx / ldc), c = static_cast<int>(idx % ldc);
+    const int i = cmap[c];
+    dst[idx] = (m < nbf && i >= 0) ? static_cast<T>(src[static_cast<size_t>(m) * ldsrc + i]) : T(0);
+}
+
+// C [nbf][ld] FP64 -> transposed padded [ldc][nbfp] BF16 hi/lo, with the column map
+__global__ void k_pack_cT_bf(const double* __restrict__ src, int ldsrc, int nbf, const int* __restrict__ cmap,
+                             uint16_t* __restrict__ hi, uint16_t* __restrict__ lo, int nbfp, int ldc) {
+    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<size_t>(nbfp) * ldc) return;
+    const int c = static_cast<int>(idx / nbfp), m = static_cast<int>(idx % nbfp);
+    const int i = cmap[c];
+    const float x = (m < nbf && i >= 0) ? static_cast<float>(src[static_cast<size_t>(m) * ldsrc + i]) : 0.0f;
+    const uint16_t h = f2bf(x);
+    hi[idx] = h;
+    lo[idx] = f2bf(x - bf2f(h));
+}
+
+// Column signs: s[i] = +1 if Cr[:,i] == Cl[:,i], -1 if Cr[:,i] == -Cl[:,i], 0 otherwise
+// (only bitwise-exact matches count).  One block per column.
+__global__ void k_column_signs(const double* __restrict__ Cl, const double* __restrict__ Cr, int ld, int nbf,
+                               int8_t* __restrict__ s) {
+    const int i = blockIdx.x;
+    __shared__ int plus, minus;
+    if (threadIdx.x == 0) {
+        plus = 1;
+        minus = 1;
+    }
+    __syncthreads();
+    int p = 1, mi = 1;
+    for (int m = threadIdx.x; m < nbf; m += blockDim.x) {
+        const double a = Cl[static_cast<size_t>(m) * ld + i], b = Cr[static_cast<size_t>(m) * ld + i];
+        p &= (a == b);
+        mi &= (a == -b);
+    }
+    if (!p) atomicAnd(&plus, 0);
+    if (!mi) atomicAnd(&minus, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) s[i] = plus ? 1 : (minus ? -1 : 0);
+}
+
+inline unsigned nblocks(size_t n, int bs) {
+    const size_t b = (n + bs - 1) / bs;
+    return static_cast<unsigned>(b > 65535u * 16 ? 65535u * 16 : b);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+void ht_panel_shape(int ncol, int* nbw, int* npanel, int* ldc) {
+    const int nbt = (ncol + 63) / 64;  // 64-column units
+    const int np = (nbt + 4) / 5;
+    const int w = (nbt + np - 1) / np;
+    *nbw = w;
+    *npanel = np;
+    *ldc = np * 64 * w;
+}
+
+template <typename T, int NBW>
+static void ht_mfma_nbw(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int npanel, T* X,
+                        int xrows, hipStream_t s) {
+    dim3 grid((g.nT + 3) / 4, nq, npanel);
+    hipLaunchKernelGGL((k_ht_mfma<T, NBW>), grid, dim3(512), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, C, ldc, X, xrows);
+}
+
+template <typename T>
+static void ht_mfma(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int nbw, int npanel, T* X,
+                    int xrows, hipStream_t s) {
+    require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 &&
+                xrows == (g.nT + 3) / 4 * 128,
+            "ht_mfma shape");
+    switch (nbw) {
+        case 1: ht_mfma_nbw<T, 1>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
+        case 2: ht_mfma_nbw<T, 2>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
+        case 3: ht_mfma_nbw<T, 3>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
+        case 4: ht_mfma_nbw<T, 4>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
+        default: ht_mfma_nbw<T, 5>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
+    }
+}
+
+void launch_ht2_f64(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int nbw, int npanel,
+                    double* X, int xrows, hipStream_t s) {
+    ht_mfma<double>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, s);
+}
+void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const float* C, int ldc, int nbw, int npanel,
+                    float* X, int xrows, hipStream_t s) {
+    ht_mfma<float>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, s);
+}
+
+void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
+                   const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
+                   hipStream_t s) {
+    require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 && g.nbfp % 32 == 0 &&
+                xrows == (g.nT + 3) / 4 * 128,
+            "ht_bf3 shape");
+    dim3 grid((g.nT + 3) / 4, nq, npanel);
+    switch (nbw) {
+#define P4A_BF3(n)                                                                                                  \
+    case n:                                                                                                         \
+        hipLaunchKernelGGL((k_ht_bf3<n>), grid, dim3(512), 0, s, Bh, Bl, g.d_tidx, g.nT, g.ntp, q0, CTh, CTl, g.nbfp, \
+                           ldc, Xh, Xl, xrows);                                                                     \
+        break;
+        P4A_BF3(1) P4A_BF3(2) P4A_BF3(3) P4A_BF3(4) P4A_BF3(5)
+#undef P4A_BF3
+    }
+}
+
+static void kgemm_check(int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, int ntile, int ks) {
+    require(ntile > 0 && nsplit > 0 && nbatch >= nsplit && xrows % 128 == 0 && kbeg >= 0 && kend <= ld &&
+                kbeg < kend && kbeg % ks == 0 && kend % ks == 0,
+            "kgemm shape");
+}
+
+void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrows, int ld, int kbeg, int kend,
+                       int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
+                       hipStream_t s) {
+    kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
+    hipLaunchKernelGGL((k_kgemm_mfma<double>), dim3(ntile, nsplit), dim3(256), 0, s, A, stride, B, stride, ld, kbeg,
+                       kend, nbatch, tiles, ntile, P, alpha, accumulate);
+}
+void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int ld, int kbeg, int kend, int nbatch,
+                       int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
+                       hipStream_t s) {
+    kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
+    hipLaunchKernelGGL((k_kgemm_mfma<float>), dim3(ntile, nsplit), dim3(256), 0, s, A, stride, B, stride, ld, kbeg,
+                       kend, nbatch, tiles, ntile, P, alpha, accumulate);
+}
+void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh, const uint16_t* Bl, size_t stride,
+                      int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles, int ntile,
+                      double* P, double alpha, int accumulate, hipStream_t s) {
+    kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 32);
+    hipLaunchKernelGGL(k_kgemm_bf3, dim3(ntile, nsplit), dim3(256), 0, s, Ah, Al, Bh, Bl, stride, ld, kbeg, kend,
+                       nbatch, tiles, ntile, P, alpha, accumulate);
+}
+
+void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n);
+}
+void launch_to_bf_split(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_bf_split, dim3(nblocks(n, 256)), dim3(256), 0, s, src, hi, lo, n);
+}
+void launch_pack_cmap_f64(const double* src, int ldsrc, int nbf, const int* cmap, double* dst, int nbfp, int ldc,
+                          hipStream_t s) {
+    const size_t n = static_cast<size_t>(nbfp) * ldc;
+    hipLaunchKernelGGL((k_pack_cmap<double>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, src, ldsrc,
+                       nbf, cmap, dst, nbfp, ldc);
+}
+void launch_pack_cmap_f32(const double* src, int ldsrc, int nbf, const int* cmap, float* dst, int nbfp, int ldc,
+                          hipStream_t s) {
+    const size_t n = static_cast<size_t>(nbfp) * ldc;
+    hipLaunchKernelGGL((k_pack_cmap<float>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, src, ldsrc,
+                       nbf, cmap, dst, nbfp, ldc);
+}
+void launch_pack_cT_bf(const double* src, int ldsrc, int nbf, const int* cmap, uint16_t* hi, uint16_t* lo, int nbfp,
+                       int ldc, hipStream_t s) {
+    const size_t n = static_cast<size_t>(nbfp) * ldc;
+    hipLaunchKernelGGL(k_pack_cT_bf, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, src, ldsrc, nbf,
+                       cmap, hi, lo, nbfp, ldc);
+}
+void launch_column_signs(const double* Cl, const double* Cr, int ld, int nbf, int ncol, int8_t* s, hipStream_t st) {
+    require(ncol > 0 && ld >= ncol, "column_signs");
+    hipLaunchKernelGGL(k_column_signs, dim3(ncol), dim3(256), 0, st, Cl, Cr, ld, nbf, s);
+}
+
+}  // namespace p4a

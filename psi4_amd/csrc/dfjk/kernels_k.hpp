@@ -1,0 +1,50 @@
+// Launchers for the pipelined K-path kernels (kernels_k.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace p4a {
+
+// Column panels of the half-transform: 64*nbw columns per panel (nbw <= 5), npanel panels, ldc = npanel*64*nbw.
+void ht_panel_shape(int ncol, int* nbw, int* npanel, int* ldc);
+
+// X[q][m][i] = sum_n B_{q0+q}[m][n] C[n][i]; C [nbfp][ldc], X [nq][xrows][ldc] with
+// xrows = 128*ceil(nT/4); every row of X is written (rows past nbf are zero).
+void launch_ht2_f64(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int nbw, int npanel,
+                    double* X, int xrows, hipStream_t s);
+void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const float* C, int ldc, int nbw, int npanel,
+                    float* X, int xrows, hipStream_t s);
+// 3-term BF16: B hi/lo [naux][ntp][32][32], C^T hi/lo [ldc][nbfp], X hi/lo [nq][xrows][ldc].
+void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
+                   const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
+                   hipStream_t s);
+
+// P[split][tile] (+)= alpha * sum_b A_b[:, kbeg:kend] B_b[:, kbeg:kend]^T over 128x128 tiles;
+// A_b = A + b*stride, rows of length ld, xrows rows (a multiple of 128).
+void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrows, int ld, int kbeg, int kend,
+                       int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
+                       hipStream_t s);
+void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int ld, int kbeg, int kend, int nbatch,
+                       int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
+                       hipStream_t s);
+void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh, const uint16_t* Bl, size_t stride,
+                      int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles, int ntile,
+                      double* P, double alpha, int accumulate, hipStream_t s);
+
+// Precision copies and operand packing (cmap: destination column -> source column or -1).
+void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
+void launch_to_bf_split(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s);
+void launch_pack_cmap_f64(const double* src, int ldsrc, int nbf, const int* cmap, double* dst, int nbfp, int ldc,
+                          hipStream_t s);
+void launch_pack_cmap_f32(const double* src, int ldsrc, int nbf, const int* cmap, float* dst, int nbfp, int ldc,
+                          hipStream_t s);
+void launch_pack_cT_bf(const double* src, int ldsrc, int nbf, const int* cmap, uint16_t* hi, uint16_t* lo, int nbfp,
+                       int ldc, hipStream_t s);
+// s[i] = +1 / -1 if Cr[:,i] is bitwise +/- Cl[:,i], else 0.
+void launch_column_signs(const double* Cl, const double* Cr, int ld, int nbf, int ncol, int8_t* s, hipStream_t st);
+
+}  // namespace p4a
