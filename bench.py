@@ -115,19 +115,24 @@ def main():
             f"J {tsum['J'] / args.steps:.2f} HT {tsum['HT'] / args.steps:.2f} KGEMM {tsum['KGEMM'] / args.steps:.2f}")
     jk.set_profiling(False)
 
-    # Incremental build (the way the mixed path is used inside SCF): K(dD), J(dD) with
-    # C_left=[C_new, C_old], C_right=[C_new, -C_old] added to FP64 J/K of D_old.
+    # Incremental build (the way the mixed path is used inside SCF): J(dD), K(dD) added to
+    # FP64 J/K of D_old, in the stable form dD = C_new dC^T + dC C_old^T with dC = C_new - C_old
+    # after aligning C_old to C_new (orthogonal Procrustes), i.e. C_left=[C_new, dC],
+    # C_right=[dC, C_old]: every low-precision product carries an O(|dC|) factor.
     inc = None
     if not args.no_incremental:
         rng = np.random.default_rng(SYNTHETIC["seed_C"] + 1)
         Cold_h = np.linalg.qr(C_h + 1e-4 * rng.standard_normal(C_h.shape))[0]
+        u, _, vt = np.linalg.svd(Cold_h.T @ C_h)
+        Cold_h = Cold_h @ (u @ vt)
+        dC_h = C_h - Cold_h
         Cold = torch.from_numpy(np.ascontiguousarray(Cold_h)).to(dev)
         Jold = torch.empty_like(D)
         Kold = torch.empty_like(D)
         jk.set_precision(PREC_FP64)
         jk.compute_device([Cold.data_ptr()], [], [nocc], [], [Jold.data_ptr()], [Kold.data_ptr()], False)
-        Cl = torch.cat([C, Cold], dim=1).contiguous()
-        Cr = torch.cat([C, -Cold], dim=1).contiguous()
+        Cl = torch.from_numpy(np.ascontiguousarray(np.hstack([C_h, dC_h]))).to(dev)
+        Cr = torch.from_numpy(np.ascontiguousarray(np.hstack([dC_h, Cold_h]))).to(dev)
         Ji = torch.empty_like(D)
         Ki = torch.empty_like(D)
         jk.set_precision(prec_id[headline])
@@ -145,9 +150,10 @@ def main():
         torch.cuda.synchronize()
         dt_inc = (time.perf_counter() - t0) / args.steps
         E_inc = float(torch.sum(D * (2.0 * Ji - Ki)).item()) if "fp64" in results else None
-        dD = float((C @ C.T - Cold @ Cold.T).abs().max().item())
+        dD = float(np.abs(C_h @ C_h.T - Cold_h @ Cold_h.T).max())
         inc = {"builds_per_s": round(1.0 / dt_inc, 4), "ms_per_build": round(dt_inc * 1e3, 3),
-               "precision": headline, "max_abs_dD": dD,
+               "precision": headline, "form": "C_left=[C_new,dC], C_right=[dC,C_old]", "max_abs_dD": dD,
+               "max_abs_dC": float(np.abs(dC_h).max()),
                "dE_vs_fp64": (abs(E_inc - results["fp64"]["E"]) if E_inc is not None else None)}
         log(f"incremental {headline}: {dt_inc * 1e3:.2f} ms/build, |dD|max={dD:.2e}, dE={inc['dE_vs_fp64']}")
         jk.set_precision(prec_id[headline])

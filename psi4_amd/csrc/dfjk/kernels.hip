@@ -192,7 +192,9 @@ __global__ void k_tile_reduce(const double* __restrict__ P, int nsplit, int ntil
     const int m = blockIdx.y;
     if (n >= N) return;
     int Ib = m >> 7, Jb = n >> 7, r = m & 127, c = n & 127;
-    if (sym && Ib > Jb) {
+    // symmetric: read the upper triangle only (also inside diagonal tiles), so the
+    // result is exactly symmetric whatever the accumulation order of the tile
+    if (sym && (Ib > Jb || (Ib == Jb && r > c))) {
         int tmp = Ib;
         Ib = Jb;
         Jb = tmp;

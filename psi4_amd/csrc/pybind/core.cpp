@@ -125,7 +125,6 @@ class JK {
 
     // JK::compute (libfock/jk.cc:595-681): lr_symmetric if C_right is empty.
     void compute() {
-        if (!initialized_) throw std::runtime_error("JK: initialize() must be called before compute()");
         const bool lr_symmetric = py::len(Cl_) > 0 && py::len(Cr_) == 0;
         const size_t n = py::len(Cl_);
         if (!lr_symmetric && py::len(Cr_) != n) throw std::runtime_error("JK: C_left/C_right irrep mismatch!");
@@ -144,6 +143,7 @@ class JK {
                 cr.push_back(R);
             }
         }
+        if (!initialized_) throw std::runtime_error("JK: initialize() must be called before compute()");
         // outputs are (re)allocated unless accumulating into existing ones of the right count
         if (!accumulate_ || J_.size() != n) {
             D_.clear();

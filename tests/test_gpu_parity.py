@@ -132,23 +132,26 @@ def test_mixed_precision_symmetric_h2o8(h2o8, prec):
 
 @pytest.mark.parametrize("prec", [1, 2])
 def test_mixed_precision_incremental_h2o8(h2o8, prec):
-    """Mixed-precision K(dD) error scales with |dD| (the reason for incremental builds)."""
+    """Mixed-precision K(dD) in the stable incremental form
+    dD = C_new dC^T + dC C_old^T (C_left=[C_new, dC], C_right=[dC, C_old], dC = C_new - C_old):
+    every product carries an O(|dC|) factor, so the error scales with |dD|."""
     mol, prim, aux, ora, jk = h2o8
     nbf = prim.nbf()
     Cn = random_orbitals(nbf, 40, seed=21)
-    Co = Cn + 1e-4 * random_orbitals(nbf, 40, seed=22)
-    announce(f"mixed precision {prec} signed incremental (H2O)_8")
+    dC = 1e-4 * random_orbitals(nbf, 40, seed=22)
+    Co = Cn - dC
+    announce(f"mixed precision {prec} stable incremental (H2O)_8")
     jk.set_precision(prec)
     try:
-        D, J, K = _run(jk, [np.hstack([Cn, Co])], [np.hstack([Cn, -Co])])
+        D, J, K = _run(jk, [np.hstack([Cn, dC])], [np.hstack([dC, Co])])
     finally:
         jk.set_precision(0)
-    _, _, Kn = ora.compute([Cn])
-    _, _, Kold = ora.compute([Co])
+    _, Jn, Kn = ora.compute([Cn])
+    _, Jold, Kold = ora.compute([Co])
     ref = Kn[0] - Kold[0]
-    scale = np.abs(Kn[0]).max()
-    # absolute error relative to the full K is far below the full-build bar
-    assert maxabs(K[0], ref) < REL_K[prec] * scale * 1e-2
+    assert maxabs(J[0], Jn[0] - Jold[0]) < TOL
+    # error relative to |dK| stays at the precision's relative level
+    assert maxabs(K[0], ref) < 5 * REL_K[prec] * np.abs(ref).max()
 
 
 def test_accumulate_and_aux_range_h2o4(h2o4):
