@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--naux", type=int, default=None)
     ap.add_argument("--nocc", type=int, default=None)
     ap.add_argument("--skip-fp64", action="store_true", help="do not time the FP64 reference build")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a device copy of this many GiB (HBM counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -59,6 +61,18 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_probe > 0:
+        n = int(args.copy_probe * (1 << 30)) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+                          "launches": 3}), flush=True)
+        return
 
     from psi4_amd.core import JK, PREC_BF16X3, PREC_FP32, PREC_FP64
     from psi4_amd.workloads import SYNTHETIC, jk_flops, synthetic_C

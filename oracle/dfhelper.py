@@ -229,17 +229,35 @@ def build_JK(Ppq, sp: Sparsity, C_left, C_right=None, do_J=True, do_K=True):
 # --------------------------------------------------------------------- driver
 class MemDFJKOracle:
     """MemDFJK restated end to end: preiterations (MemDFJK.cc:71-96 ->
-    DFHelper::initialize, dfhelper.cc:149-215) then per-call build_JK."""
+    DFHelper::initialize, dfhelper.cc:149-215) then per-call build_JK.
 
-    def __init__(self, primary, aux, cutoff=1e-12, condition=1e-10, nthreads=None):
+    fit="eigen" is psi4's J^-1/2 with the condition cut (matrix.cc:2370-2424);
+    fit="cholesky" is B = L^-1 (P|mn) with J = L L^T (the device library's
+    factorisation).  With B given (dense [naux, nbf, nbf]), the fit is skipped
+    and the contractions run on that tensor, so J/K parity of the device path
+    can be checked on identical B."""
+
+    def __init__(self, primary, aux, cutoff=1e-12, condition=1e-10, nthreads=None, fit="eigen", B=None):
         from psi4_amd import ints  # the Libint2 stand-in (host integral engine)
         self.primary, self.aux = primary, aux
         self.cutoff, self.condition = cutoff, condition
         self.nbf, self.naux = primary.nbf(), aux.nbf()
         fm, sm, mx = ints.schwarz(primary, nthreads)
         self.sp = prepare_sparsity(fm, sm, mx, cutoff, self.naux)
+        if B is not None:
+            self.Ppq = ppq_from_dense(np.asarray(B), self.sp)
+            self.metric = self.metp = self.Pmn = None
+            return
         self.metric = ints.metric(aux, nthreads)
-        self.metp, self.n_kept, self.metric_eigs = metric_power(self.metric, -0.5, condition)
+        if fit == "eigen":
+            self.metp, self.n_kept, self.metric_eigs = metric_power(self.metric, -0.5, condition)
+        elif fit == "cholesky":
+            from scipy.linalg import solve_triangular
+            L = np.linalg.cholesky(self.metric)
+            self.metp = solve_triangular(L, np.eye(self.naux), lower=True)
+            self.n_kept, self.metric_eigs = self.naux, None
+        else:
+            raise ValueError(fit)
         self.Pmn = ints.three_index(primary, aux, nthreads)
         # dropped pairs contribute nothing (psi4 never computes them)
         self.Pmn[:, ~self.sp.fun_mask] = 0.0
@@ -253,3 +271,12 @@ class MemDFJKOracle:
 
     def compute(self, C_left, C_right=None, do_J=True, do_K=True):
         return build_JK(self.Ppq, self.sp, C_left, C_right, do_J, do_K)
+
+
+def ppq_from_dense(B, sp: Sparsity):
+    """pQq layout (dfhelper.cc:1653-1678 storage) from a dense B [naux, nbf, nbf]."""
+    out = np.zeros(int(sp.big_skips[sp.nbf]))
+    for k in range(sp.nbf):
+        cols = np.nonzero(sp.fun_mask[k])[0]
+        out[sp.big_skips[k]:sp.big_skips[k + 1]] = B[:, k, cols].reshape(-1)
+    return out
