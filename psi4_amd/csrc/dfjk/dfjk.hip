@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -170,6 +171,7 @@ struct dfjk_handle {
     int ntile_sym = 0, ntile_all = 0;
     // resident tensor and its lower-precision copies
     DevBuf dB, dB32, dBh, dBl;
+    DevBuf dZero;  // one zero TILE x TILE FP64 block, read in place of absent blocks
     // workspace
     DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G, ws_sign, ws_c32, ws_cbf, ws_cmap;
     // timing
@@ -251,6 +253,8 @@ void build_tiles(dfjk_handle* h) {
 }
 
 void upload_tiles(dfjk_handle* h) {
+    hcheck(hipMemset(h->dZero.get<double>(TILE2), 0, TILE2 * sizeof(double)), "zero tile");
+    if (const char* e = std::getenv("PSI4AMD_HT_ROWWAVES")) set_ht_rowwaves(std::atoi(e));
     const auto& sym = h->otiles_sym;
     const auto& all = h->otiles_all;
     const auto& of_sym = h->otile_of_sym;
@@ -363,9 +367,12 @@ void initialize_basis(dfjk_handle* h) {
 }
 
 // ---------------------------------------------------------------- compute
+// Q-splits of the contraction: about 2048 workgroups, a multiple of the 8 XCDs when
+// the batch allows (the contraction maps each split to one XCD).
 int choose_nsplit(int ntile, int nbatch) {
     const int target = 2048;
     int ns = std::max(1, (target + ntile - 1) / ntile);
+    ns = (ns + 7) / 8 * 8;
     return std::min(ns, std::max(1, nbatch));
 }
 
@@ -555,22 +562,22 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             hipEvent_t e0 = tm.begin();
             if (prec == DFJK_PREC_FP64) {
                 launch_ht2_f64(g, h->dB.as<double>(), qa, nq, cl, ldc, nbw, npanel, reinterpret_cast<double*>(XL),
-                               xrows, s);
+                               xrows, h->dZero.p, s);
                 if (general)
                     launch_ht2_f64(g, h->dB.as<double>(), qa, nq, cr, ldc, nbw, npanel, reinterpret_cast<double*>(XR),
-                                   xrows, s);
+                                   xrows, h->dZero.p, s);
             } else if (prec == DFJK_PREC_FP32) {
-                launch_ht2_f32(g, B32, qa, nq, c32l, ldc, nbw, npanel, reinterpret_cast<float*>(XL), xrows, s);
+                launch_ht2_f32(g, B32, qa, nq, c32l, ldc, nbw, npanel, reinterpret_cast<float*>(XL), xrows, h->dZero.p, s);
                 if (general)
-                    launch_ht2_f32(g, B32, qa, nq, c32r, ldc, nbw, npanel, reinterpret_cast<float*>(XR), xrows, s);
+                    launch_ht2_f32(g, B32, qa, nq, c32r, ldc, nbw, npanel, reinterpret_cast<float*>(XR), xrows, h->dZero.p, s);
             } else {
                 uint16_t* xl = reinterpret_cast<uint16_t*>(XL);
                 uint16_t* xr = reinterpret_cast<uint16_t*>(XR);
                 launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), qa, nq, cth_l, ctl_l, ldc, nbw, npanel,
-                              xl, xl + xq * qblk, xrows, s);
+                              xl, xl + xq * qblk, xrows, h->dZero.p, s);
                 if (general)
                     launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), qa, nq, cth_r, ctl_r, ldc, nbw,
-                                  npanel, xr, xr + xq * qblk, xrows, s);
+                                  npanel, xr, xr + xq * qblk, xrows, h->dZero.p, s);
             }
             tm.end(T_HT, e0);
             h->n_ht += general ? 2 : 1;

@@ -28,24 +28,6 @@ __device__ inline uint16_t f2bf(float x) {
 }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
-// Row-tile lookup for a 128-row (four storage tiles) workgroup at column tile J.
-struct RowTiles {
-    int t[4];
-    bool any;
-    // select without dynamic indexing (keeps the array in registers)
-    __device__ int at(int s) const { return s == 0 ? t[0] : s == 1 ? t[1] : s == 2 ? t[2] : t[3]; }
-};
-__device__ inline RowTiles row_tiles(const int* __restrict__ tidx, int nT, int I0, int J) {
-    RowTiles rt;
-    rt.any = false;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int I = I0 + s;
-        rt.t[s] = (I < nT) ? tidx[min(I, J) * nT + max(I, J)] : -1;
-        rt.any |= rt.t[s] >= 0;
-    }
-    return rt;
-}
 // Element e of a 128 x KS chunk of A: row tile e / (32*KS), then (r, c) in the stored
 // tile's own orientation so that global reads are contiguous for both orientations.
 struct ACoord {
@@ -66,38 +48,115 @@ __device__ inline ACoord a_coord(int e, int I0, int J) {
     return a;
 }
 
-__device__ inline int next_present(const int* __restrict__ tidx, int nT, int I0, int J) {
-    while (J < nT && !row_tiles(tidx, nT, I0, J).any) ++J;
-    return J;
+// Workgroups are dispatched round-robin over the XCDs (flat id w runs on XCD w % 8,
+// each XCD with its own L2).  xcd_map turns the flat id into (group, item) so that
+// all items of one group -- the row blocks of one Q in the half-transform, the
+// output tiles of one Q-split in the contraction -- run on the same XCD and share
+// its L2.  The launch uses 8 * ceil(ngroups/8) * items workgroups.
+constexpr int kXCD = 8;
+struct XcdItem {
+    int group, item;
+    bool valid;
+};
+__device__ inline XcdItem xcd_map(int wg, int items, int ngroups) {
+    const int xcd = wg % kXCD, local = wg / kXCD;
+    XcdItem d;
+    d.group = (local / items) * kXCD + xcd;
+    d.item = local % items;
+    d.valid = d.group < ngroups;
+    return d;
+}
+inline unsigned xcd_grid(int ngroups, int items) {
+    return static_cast<unsigned>(kXCD * ((ngroups + kXCD - 1) / kXCD) * items);
+}
+
+// Per-workgroup list of the column tiles J that meet at least one of the
+// workgroup's RT row tiles, with the element offset of each (row tile, J) block
+// in B_Q (or -1), built once in LDS at kernel start (parallel presence test and
+// an inclusive scan), so the K loop does no dependent tile-table lookups.
+constexpr int kMaxColTiles = 256;  // nbf <= 8192
+
+template <int RT>
+struct StepList {
+    int off[kMaxColTiles * RT];
+    int J[kMaxColTiles];
+    int cnt[kMaxColTiles];
+    int n;
+};
+
+template <int RT>
+__device__ inline void build_steps(StepList<RT>& sl, const int* __restrict__ tidx, int nT, int I0, int tid, int nthr) {
+    for (int j = tid; j < kMaxColTiles; j += nthr) {
+        int present = 0;
+#pragma unroll
+        for (int s = 0; s < RT; ++s) {
+            const int I = I0 + s;
+            int t = -1;
+            if (j < nT && I < nT) t = tidx[min(I, j) * nT + max(I, j)];
+            sl.off[j * RT + s] = t >= 0 ? t * TILE2 : -1;
+            present |= (t >= 0);
+        }
+        sl.cnt[j] = present;
+    }
+    __syncthreads();
+    // inclusive scan of the presence flags (Hillis-Steele over kMaxColTiles entries)
+    for (int d = 1; d < kMaxColTiles; d <<= 1) {
+        int v = 0;
+        if (tid < kMaxColTiles && tid >= d) v = sl.cnt[tid - d];
+        __syncthreads();
+        if (tid < kMaxColTiles) sl.cnt[tid] += v;
+        __syncthreads();
+    }
+    // compact: step k -> J (offsets are kept indexed by J)
+    if (tid < kMaxColTiles) {
+        const int prev = tid ? sl.cnt[tid - 1] : 0;
+        if (sl.cnt[tid] != prev) sl.J[prev] = tid;
+        if (tid == kMaxColTiles - 1) sl.n = sl.cnt[tid];
+    }
+    __syncthreads();
 }
 
 // =====================================================================================
-// Half-transform, FP64 / FP32 (16x16x4 MFMA).  512 threads = 8 waves (2 rows x 4 cols),
-// workgroup tile 128 rows x 64*NBW columns, wave tile 64 x 16*NBW, K-step 16.
+// Half-transform, FP64 / FP32 (16x16x4 MFMA).  WM*256 threads = WM*4 waves (WM rows x
+// 4 cols), workgroup tile 64*WM rows x 64*NBW columns, wave tile 64 x 16*NBW.
+// K-step 8 (FP64) or 16 (FP32).  Absent tiles read a device zero tile (no branches).
 // T = double or float; B is the FP64 tensor (T=double) or its FP32 copy (T=float).
-// grid = (ceil(nT/4), nq, npanel)
+// grid = (xrows / (64*WM), nq, npanel)
 // =====================================================================================
-template <typename T, int NBW>
-__global__ __launch_bounds__(512) void k_ht_mfma(const T* __restrict__ B, const int* __restrict__ tidx, int nT,
-                                                 int ntp, int q0, const T* __restrict__ C, int ldc, T* __restrict__ X,
-                                                 int xrows) {
+template <typename T, int NBW, int WM>
+__global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ht_mfma(const T* __restrict__ B, const int* __restrict__ tidx, int nT,
+                                                      int ntp, int q0, int nq, int npanel, const T* __restrict__ C,
+                                                      int ldc, T* __restrict__ X, int xrows,
+                                                      const T* __restrict__ ztile) {
     constexpr bool F64 = sizeof(T) == 8;
+    constexpr int NTHR = WM * 256;
+    constexpr int RT = 2 * WM;            // row tiles per workgroup
+    constexpr int ROWS = 64 * WM;
     constexpr int PAN = 64 * NBW;
-    constexpr int KS = F64 ? 8 : 16;      // K-step: FP64 keeps the prefetch registers small
+    constexpr int KS = F64 ? 8 : 16;
     constexpr int AP = KS + 1;
     constexpr int CP = PAN + 4;
-    constexpr int NA = 128 * KS / 512;    // A elements per thread per step
-    constexpr int NC = KS * PAN / 512;    // C elements per thread per step
-    __shared__ T As[2][128 * AP];
+    constexpr int NA = ROWS * KS / NTHR;  // A elements per thread per step
+    constexpr int NC = KS * PAN / NTHR;   // C elements per thread per step
+    static_assert((ROWS * KS) % NTHR == 0 && (KS * PAN) % NTHR == 0, "staging shape");
+    __shared__ T As[2][ROWS * AP];
     __shared__ T Cs[2][KS * CP];
+    __shared__ StepList<RT> sl;
     typedef typename std::conditional<F64, d4_t, f4_t>::type acc_t;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 2, wn = wave & 3;
-    const int I0 = 4 * blockIdx.x;
-    const int q = blockIdx.y;
-    const int col0 = blockIdx.z * PAN;
+    const int nrb = xrows / ROWS;
+    const XcdItem xi = xcd_map(blockIdx.x, nrb * npanel, nq);
+    if (!xi.valid) return;
+    const int rb = xi.item / npanel;
+    const int I0 = RT * rb;
+    const int q = xi.group;
+    const int col0 = (xi.item % npanel) * PAN;
     const T* BQ = B + static_cast<size_t>(q0 + q) * ntp * TILE2;
+
+    build_steps<RT>(sl, tidx, nT, I0, tid, NTHR);
+    const int nsteps = sl.n * (TILE / KS);
 
     acc_t acc[4][NBW];
 #pragma unroll
@@ -106,36 +165,34 @@ __global__ __launch_bounds__(512) void k_ht_mfma(const T* __restrict__ B, const 
         for (int j = 0; j < NBW; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     T ra[NA], rc[NC];
-    auto gload = [&](int J, int kh) {
-        const RowTiles rt = row_tiles(tidx, nT, I0, J);
+    auto gload = [&](int st) {
+        const int J = sl.J[st / (TILE / KS)];
+        const int kh = (st % (TILE / KS)) * KS;
 #pragma unroll
         for (int it = 0; it < NA; ++it) {
-            const ACoord ac = a_coord<KS>(tid + 512 * it, I0, J);
-            const int t = rt.at(ac.sub);
-            T v = 0;
-            if (t >= 0) {
-                const T* tp = BQ + static_cast<size_t>(t) * TILE2;
-                v = (I0 + ac.sub <= J) ? tp[ac.r * TILE + kh + ac.c] : tp[(kh + ac.c) * TILE + ac.r];
-            }
-            ra[it] = v;
+            const ACoord ac = a_coord<KS>(tid + NTHR * it, I0, J);
+            const int off = sl.off[J * RT + ac.sub];
+            const T* tp = off >= 0 ? BQ + off : ztile;
+            ra[it] = (I0 + ac.sub <= J) ? tp[ac.r * TILE + kh + ac.c] : tp[(kh + ac.c) * TILE + ac.r];
         }
         const T* Cj = C + static_cast<size_t>(J * TILE + kh) * ldc + col0;
 #pragma unroll
         for (int it = 0; it < NC; ++it) {
-            const int e = tid + 512 * it;
+            const int e = tid + NTHR * it;
             const int kk = e / PAN, cc = e - kk * PAN;
             rc[it] = Cj[static_cast<size_t>(kk) * ldc + cc];
         }
     };
-    auto lstore = [&](int buf, int J) {
+    auto lstore = [&](int buf, int st) {
+        const int J = sl.J[st / (TILE / KS)];
 #pragma unroll
         for (int it = 0; it < NA; ++it) {
-            const ACoord ac = a_coord<KS>(tid + 512 * it, I0, J);
+            const ACoord ac = a_coord<KS>(tid + NTHR * it, I0, J);
             As[buf][(ac.sub * 32 + ac.r) * AP + ac.c] = ra[it];
         }
 #pragma unroll
         for (int it = 0; it < NC; ++it) {
-            const int e = tid + 512 * it;
+            const int e = tid + NTHR * it;
             const int kk = e / PAN, cc = e - kk * PAN;
             Cs[buf][kk * CP + cc] = rc[it];
         }
@@ -160,32 +217,21 @@ __global__ __launch_bounds__(512) void k_ht_mfma(const T* __restrict__ B, const 
         }
     };
 
-    int J = next_present(tidx, nT, I0, 0);
-    int kh = 0;
-    if (J < nT) {
-        gload(J, kh);
-        lstore(0, J);
+    if (nsteps > 0) {
+        gload(0);
+        lstore(0, 0);
         __syncthreads();
         int buf = 0;
-        while (true) {
-            int nJ = J, nkh = kh + KS;
-            if (nkh == TILE) {
-                nkh = 0;
-                nJ = next_present(tidx, nT, I0, J + 1);
-            }
-            const bool more = nJ < nT;
-            if (more) gload(nJ, nkh);
+        for (int st = 0; st < nsteps; ++st) {
+            const bool more = st + 1 < nsteps;
+            if (more) gload(st + 1);
             compute(buf);
-            if (more) lstore(buf ^ 1, nJ);
+            if (more) lstore(buf ^ 1, st + 1);
             __syncthreads();
-            if (!more) break;
             buf ^= 1;
-            J = nJ;
-            kh = nkh;
         }
     }
-    // every row of the 128-row block is written (absent row tiles give zeros), so the
-    // X buffer needs no padding fill
+    // every row of the workgroup's block is written (absent row tiles give zeros)
     T* Xq = X + static_cast<size_t>(q) * xrows * ldc;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -194,7 +240,7 @@ __global__ __launch_bounds__(512) void k_ht_mfma(const T* __restrict__ B, const 
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int rr = F64 ? (4 * r + (lane >> 4)) : (4 * (lane >> 4) + r);
-                const int m = 128 * blockIdx.x + wm * 64 + i * 16 + rr;
+                const int m = ROWS * rb + wm * 64 + i * 16 + rr;
                 Xq[static_cast<size_t>(m) * ldc + col0 + wn * NBW * 16 + j * 16 + (lane & 15)] = acc[i][j][r];
             }
 }
@@ -202,28 +248,45 @@ __global__ __launch_bounds__(512) void k_ht_mfma(const T* __restrict__ B, const 
 // =====================================================================================
 // Half-transform, 3-term BF16 split: X = Bh Ch + Bh Cl + Bl Ch with FP32 accumulation.
 // B as hi/lo BF16 copies, C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one
-// column tile (32).  Output X split again into hi/lo BF16 ([nq][nbfp][ldc]).
+// column tile (32).  WM*256 threads, workgroup tile 64*WM rows x 64*NBW columns.
+// Output X split again into hi/lo BF16 ([nq][xrows][ldc]).
 // =====================================================================================
-template <int NBW>
-__global__ __launch_bounds__(512) void k_ht_bf3(const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl,
-                                                const int* __restrict__ tidx, int nT, int ntp, int q0,
-                                                const uint16_t* __restrict__ CTh, const uint16_t* __restrict__ CTl,
-                                                int nbfp, int ldc, uint16_t* __restrict__ Xh, uint16_t* __restrict__ Xl,
-                                                int xrows) {
+template <int NBW, int WM>
+__global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ht_bf3(const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl,
+                                                     const int* __restrict__ tidx, int nT, int ntp, int q0, int nq,
+                                                     int npanel, const uint16_t* __restrict__ CTh,
+                                                     const uint16_t* __restrict__ CTl,
+                                                     int nbfp, int ldc, uint16_t* __restrict__ Xh,
+                                                     uint16_t* __restrict__ Xl, int xrows,
+                                                     const uint16_t* __restrict__ ztile) {
+    constexpr int NTHR = WM * 256;
+    constexpr int RT = 2 * WM;
+    constexpr int ROWS = 64 * WM;
     constexpr int PAN = 64 * NBW;
-    constexpr int KP = 40;                // LDS pitch in bf16 (80 B rows, 16 B aligned)
-    constexpr int A_SZ = 128 * KP;
+    constexpr int KP = 40;                   // LDS pitch in bf16 (80 B rows, 16 B aligned)
+    constexpr int A_SZ = ROWS * KP;
     constexpr int C_SZ = PAN * KP;
-    constexpr int NCV = PAN * 4 / 512;    // 16-byte C vectors per thread per array (PAN*32/8/512)
-    constexpr int NCV_R = (PAN * 4) % 512;
+    constexpr int NAV = ROWS * 4 / NTHR;     // 16-byte A vectors per thread per array (= 1)
+    constexpr int NCV = PAN * 4 / NTHR;      // full rounds of 16-byte C vectors per thread per array
+    constexpr int NCV_R = (PAN * 4) % NTHR;  // vectors in the last partial round
+    static_assert(NAV == 1, "one A vector per thread");
+    static_assert(NCV + (NCV_R ? 1 : 0) <= 5, "C staging slots");
     __shared__ uint16_t Ls[2][2 * A_SZ + 2 * C_SZ];
+    __shared__ StepList<RT> sl;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 2, wn = wave & 3;
-    const int I0 = 4 * blockIdx.x;
-    const int q = blockIdx.y;
-    const int col0 = blockIdx.z * PAN;
+    const int nrb = xrows / ROWS;
+    const XcdItem xi = xcd_map(blockIdx.x, nrb * npanel, nq);
+    if (!xi.valid) return;
+    const int rb = xi.item / npanel;
+    const int I0 = RT * rb;
+    const int q = xi.group;
+    const int col0 = (xi.item % npanel) * PAN;
     const size_t qoff = static_cast<size_t>(q0 + q) * ntp * TILE2;
+
+    build_steps<RT>(sl, tidx, nT, I0, tid, NTHR);
+    const int nsteps = sl.n;
 
     f4_t acc[4][NBW];
 #pragma unroll
@@ -231,47 +294,42 @@ __global__ __launch_bounds__(512) void k_ht_bf3(const uint16_t* __restrict__ Bh,
 #pragma unroll
         for (int j = 0; j < NBW; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
 
-    // A: per row tile 32x32 bf16 = 128 vectors of 8; 4 row tiles -> 512 vectors per array.
-    static_assert(NCV + (NCV_R ? 1 : 0) <= 3, "C staging slots");
+    // A: per row tile 32x32 bf16 = 128 vectors of 8; RT row tiles -> one vector per thread per array.
+    const int sub = tid >> 7, f = tid & 127;
     uint4 rah, ral;
-    uint4 c0h, c0l, c1h, c1l, c2h, c2l;
-    int at_flag;  // transposed?
-    auto gload = [&](int J) {
-        const RowTiles rt = row_tiles(tidx, nT, I0, J);
-        const int sub = tid >> 7, f = tid & 127;
-        const int t = rt.at(sub);
-        const int I = I0 + sub;
-        at_flag = (I > J);
-        rah = make_uint4(0, 0, 0, 0);
-        ral = rah;
-        if (t >= 0) {
-            const uint16_t* th = Bh + qoff + static_cast<size_t>(t) * TILE2;
-            const uint16_t* tl = Bl + qoff + static_cast<size_t>(t) * TILE2;
-            // vector f covers row (f>>2), cols 8*(f&3)..+7 of the stored tile
-            const int off = (f >> 2) * TILE + 8 * (f & 3);
-            rah = *reinterpret_cast<const uint4*>(th + off);
-            ral = *reinterpret_cast<const uint4*>(tl + off);
-        }
+    uint4 c0h, c0l, c1h, c1l, c2h, c2l, c3h, c3l, c4h, c4l;
+    int at_flag = 0;
+    auto gload = [&](int st) {
+        const int J = sl.J[st];
+        const int off = sl.off[J * RT + sub];
+        at_flag = (I0 + sub > J);
+        const uint16_t* th = off >= 0 ? Bh + qoff + off : ztile;
+        const uint16_t* tl = off >= 0 ? Bl + qoff + off : ztile;
+        // vector f covers stored-tile row (f>>2), cols 8*(f&3)..+7
+        const int o = (f >> 2) * TILE + 8 * (f & 3);
+        rah = *reinterpret_cast<const uint4*>(th + o);
+        ral = *reinterpret_cast<const uint4*>(tl + o);
         const size_t kb = static_cast<size_t>(J) * TILE;
         auto cload = [&](int it, uint4& h, uint4& l) {
-            const int v = tid + 512 * it;
-            if (it < NCV || v < NCV_R + 512 * NCV) {
+            const int v = tid + NTHR * it;
+            if (it < NCV || v < NCV_R + NTHR * NCV) {
                 const int col = v >> 2, k8 = 8 * (v & 3);
-                const size_t off = static_cast<size_t>(col0 + col) * nbfp + kb + k8;
-                h = *reinterpret_cast<const uint4*>(CTh + off);
-                l = *reinterpret_cast<const uint4*>(CTl + off);
+                const size_t co = static_cast<size_t>(col0 + col) * nbfp + kb + k8;
+                h = *reinterpret_cast<const uint4*>(CTh + co);
+                l = *reinterpret_cast<const uint4*>(CTl + co);
             }
         };
         cload(0, c0h, c0l);
         if (NCV >= 1) cload(1, c1h, c1l);
         if (NCV >= 2) cload(2, c2h, c2l);
+        if (NCV >= 3) cload(3, c3h, c3l);
+        if (NCV >= 4) cload(4, c4h, c4l);
     };
     auto lstore = [&](int buf) {
         uint16_t* Ah = Ls[buf];
         uint16_t* Al = Ah + A_SZ;
         uint16_t* Ch = Al + A_SZ;
         uint16_t* Cl = Ch + C_SZ;
-        const int sub = tid >> 7, f = tid & 127;
         const int r = f >> 2, c8 = 8 * (f & 3);
         if (!at_flag) {
             *reinterpret_cast<uint4*>(Ah + (sub * 32 + r) * KP + c8) = rah;
@@ -288,8 +346,8 @@ __global__ __launch_bounds__(512) void k_ht_bf3(const uint16_t* __restrict__ Bh,
             }
         }
         auto cstore = [&](int it, const uint4& h, const uint4& l) {
-            const int v = tid + 512 * it;
-            if (it < NCV || v < NCV_R + 512 * NCV) {
+            const int v = tid + NTHR * it;
+            if (it < NCV || v < NCV_R + NTHR * NCV) {
                 const int col = v >> 2, k8 = 8 * (v & 3);
                 *reinterpret_cast<uint4*>(Ch + col * KP + k8) = h;
                 *reinterpret_cast<uint4*>(Cl + col * KP + k8) = l;
@@ -298,6 +356,8 @@ __global__ __launch_bounds__(512) void k_ht_bf3(const uint16_t* __restrict__ Bh,
         cstore(0, c0h, c0l);
         if (NCV >= 1) cstore(1, c1h, c1l);
         if (NCV >= 2) cstore(2, c2h, c2l);
+        if (NCV >= 3) cstore(3, c3h, c3l);
+        if (NCV >= 4) cstore(4, c4h, c4l);
     };
     auto compute = [&](int buf) {
         const uint16_t* Ah = Ls[buf];
@@ -326,22 +386,18 @@ __global__ __launch_bounds__(512) void k_ht_bf3(const uint16_t* __restrict__ Bh,
         }
     };
 
-    int J = next_present(tidx, nT, I0, 0);
-    if (J < nT) {
-        gload(J);
+    if (nsteps > 0) {
+        gload(0);
         lstore(0);
         __syncthreads();
         int buf = 0;
-        while (true) {
-            const int nJ = next_present(tidx, nT, I0, J + 1);
-            const bool more = nJ < nT;
-            if (more) gload(nJ);
+        for (int st = 0; st < nsteps; ++st) {
+            const bool more = st + 1 < nsteps;
+            if (more) gload(st + 1);
             compute(buf);
             if (more) lstore(buf ^ 1);
             __syncthreads();
-            if (!more) break;
             buf ^= 1;
-            J = nJ;
         }
     }
     const size_t xq = static_cast<size_t>(q) * xrows * ldc;
@@ -351,7 +407,7 @@ __global__ __launch_bounds__(512) void k_ht_bf3(const uint16_t* __restrict__ Bh,
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = 128 * blockIdx.x + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+                const int m = ROWS * rb + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
                 const size_t o = xq + static_cast<size_t>(m) * ldc + col0 + wn * NBW * 16 + j * 16 + (lane & 15);
                 const float x = acc[i][j][r];
                 const uint16_t hi = f2bf(x);
@@ -368,8 +424,8 @@ __global__ __launch_bounds__(512) void k_ht_bf3(const uint16_t* __restrict__ Bh,
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_mfma(const T* __restrict__ A, size_t strideA, const T* __restrict__ Bm,
                                                     size_t strideB, int ld, int kbeg, int kend, int nbatch,
-                                                    const int2* __restrict__ tiles, int ntile, double* __restrict__ P,
-                                                    double alpha, int accumulate) {
+                                                    int nsplit, const int2* __restrict__ tiles, int ntile,
+                                                    double* __restrict__ P, double alpha, int accumulate) {
     constexpr int KS = 16, AP = KS + 1;
     constexpr bool F64 = sizeof(T) == 8;
     typedef typename std::conditional<F64, d4_t, f4_t>::type acc_t;
@@ -377,7 +433,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __shared__ T Bs[2][128 * AP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    const int tile = blockIdx.x, split = blockIdx.y, nsplit = gridDim.y;
+    const XcdItem xi = xcd_map(blockIdx.x, ntile, nsplit);
+    if (!xi.valid) return;
+    const int tile = xi.item, split = xi.group;
     const int2 tb = tiles[tile];
     const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
     const int b1 = static_cast<int>((static_cast<long long>(nbatch) * (split + 1)) / nsplit);
@@ -471,13 +529,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 __global__ __launch_bounds__(256) void k_kgemm_bf3(const uint16_t* __restrict__ Ah_, const uint16_t* __restrict__ Al_,
                                                    const uint16_t* __restrict__ Bh_, const uint16_t* __restrict__ Bl_,
                                                    size_t stride, int ld, int kbeg, int kend, int nbatch,
-                                                   const int2* __restrict__ tiles, int ntile, double* __restrict__ P,
-                                                   double alpha, int accumulate) {
+                                                   int nsplit, const int2* __restrict__ tiles, int ntile,
+                                                   double* __restrict__ P, double alpha, int accumulate) {
     constexpr int KS = 32, KP = 40, SZ = 128 * KP;
     __shared__ uint16_t Ls[2][4 * SZ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    const int tile = blockIdx.x, split = blockIdx.y, nsplit = gridDim.y;
+    const XcdItem xi = xcd_map(blockIdx.x, ntile, nsplit);
+    if (!xi.valid) return;
+    const int tile = xi.item, split = xi.group;
     const int2 tb = tiles[tile];
     const int m0 = tb.x * 128, n0 = tb.y * 128;
     const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
@@ -656,53 +716,79 @@ void ht_panel_shape(int ncol, int* nbw, int* npanel, int* ldc) {
     *ldc = np * 64 * w;
 }
 
-template <typename T, int NBW>
+template <typename T, int NBW, int WM>
 static void ht_mfma_nbw(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int npanel, T* X,
-                        int xrows, hipStream_t s) {
-    dim3 grid((g.nT + 3) / 4, nq, npanel);
-    hipLaunchKernelGGL((k_ht_mfma<T, NBW>), grid, dim3(512), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, C, ldc, X, xrows);
+                        int xrows, const void* ztile, hipStream_t s) {
+    dim3 grid(xcd_grid(nq, xrows / (64 * WM) * npanel));
+    hipLaunchKernelGGL((k_ht_mfma<T, NBW, WM>), grid, dim3(WM * 256), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, C,
+                       ldc, X, xrows, static_cast<const T*>(ztile));
 }
 
-template <typename T>
-static void ht_mfma(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int nbw, int npanel, T* X,
-                    int xrows, hipStream_t s) {
-    require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 &&
-                xrows == (g.nT + 3) / 4 * 128,
-            "ht_mfma shape");
+template <typename T, int WM>
+static void ht_mfma_wm(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int nbw, int npanel, T* X,
+                       int xrows, const void* zt, hipStream_t s) {
     switch (nbw) {
-        case 1: ht_mfma_nbw<T, 1>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
-        case 2: ht_mfma_nbw<T, 2>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
-        case 3: ht_mfma_nbw<T, 3>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
-        case 4: ht_mfma_nbw<T, 4>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
-        default: ht_mfma_nbw<T, 5>(g, B, q0, nq, C, ldc, npanel, X, xrows, s); break;
+        case 1: ht_mfma_nbw<T, 1, WM>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
+        case 2: ht_mfma_nbw<T, 2, WM>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
+        case 3: ht_mfma_nbw<T, 3, WM>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
+        case 4: ht_mfma_nbw<T, 4, WM>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
+        default: ht_mfma_nbw<T, 5, WM>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
     }
 }
 
-void launch_ht2_f64(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int nbw, int npanel,
-                    double* X, int xrows, hipStream_t s) {
-    ht_mfma<double>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, s);
-}
-void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const float* C, int ldc, int nbw, int npanel,
-                    float* X, int xrows, hipStream_t s) {
-    ht_mfma<float>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, s);
+int g_ht_wm = 0;  // 0: default shape; 1 or 2 force the row-wave count (tuning sweeps)
+
+template <typename T>
+static void ht_mfma(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int nbw, int npanel, T* X,
+                    int xrows, const void* zt, hipStream_t s) {
+    require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 &&
+                xrows == (g.nT + 3) / 4 * 128 && g.nT <= kMaxColTiles && zt != nullptr,
+            "ht_mfma shape");
+    const int wm = g_ht_wm ? g_ht_wm : 2;
+    if (wm == 1)
+        ht_mfma_wm<T, 1>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
+    else
+        ht_mfma_wm<T, 2>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
 }
 
-void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
-                   const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
-                   hipStream_t s) {
-    require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 && g.nbfp % 32 == 0 &&
-                xrows == (g.nT + 3) / 4 * 128,
-            "ht_bf3 shape");
-    dim3 grid((g.nT + 3) / 4, nq, npanel);
+void launch_ht2_f64(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int nbw, int npanel,
+                    double* X, int xrows, const void* zt, hipStream_t s) {
+    ht_mfma<double>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
+}
+void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const float* C, int ldc, int nbw, int npanel,
+                    float* X, int xrows, const void* zt, hipStream_t s) {
+    ht_mfma<float>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
+}
+void set_ht_rowwaves(int wm) { g_ht_wm = (wm == 1 || wm == 2) ? wm : 0; }
+
+template <int WM>
+static void ht_bf3_wm(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
+                      const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
+                      const void* zt, hipStream_t s) {
+    dim3 grid(xcd_grid(nq, xrows / (64 * WM) * npanel));
+    const uint16_t* z = static_cast<const uint16_t*>(zt);
     switch (nbw) {
 #define P4A_BF3(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_ht_bf3<n>), grid, dim3(512), 0, s, Bh, Bl, g.d_tidx, g.nT, g.ntp, q0, CTh, CTl, g.nbfp, \
-                           ldc, Xh, Xl, xrows);                                                                     \
+        hipLaunchKernelGGL((k_ht_bf3<n, WM>), grid, dim3(WM * 256), 0, s, Bh, Bl, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, \
+                           CTh, CTl, g.nbfp, ldc, Xh, Xl, xrows, z);                                                \
         break;
         P4A_BF3(1) P4A_BF3(2) P4A_BF3(3) P4A_BF3(4) P4A_BF3(5)
 #undef P4A_BF3
     }
+}
+
+void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
+                   const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
+                   const void* zt, hipStream_t s) {
+    require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 && g.nbfp % 32 == 0 &&
+                xrows == (g.nT + 3) / 4 * 128 && g.nT <= kMaxColTiles && zt != nullptr,
+            "ht_bf3 shape");
+    const int wm = g_ht_wm ? g_ht_wm : 2;
+    if (wm == 1)
+        ht_bf3_wm<1>(g, Bh, Bl, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+    else
+        ht_bf3_wm<2>(g, Bh, Bl, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
 }
 
 static void kgemm_check(int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, int ntile, int ks) {
@@ -715,22 +801,22 @@ void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrow
                        int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
                        hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
-    hipLaunchKernelGGL((k_kgemm_mfma<double>), dim3(ntile, nsplit), dim3(256), 0, s, A, stride, B, stride, ld, kbeg,
-                       kend, nbatch, tiles, ntile, P, alpha, accumulate);
+    hipLaunchKernelGGL((k_kgemm_mfma<double>), dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, A, stride, B, stride, ld,
+                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate);
 }
 void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int ld, int kbeg, int kend, int nbatch,
                        int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
                        hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
-    hipLaunchKernelGGL((k_kgemm_mfma<float>), dim3(ntile, nsplit), dim3(256), 0, s, A, stride, B, stride, ld, kbeg,
-                       kend, nbatch, tiles, ntile, P, alpha, accumulate);
+    hipLaunchKernelGGL((k_kgemm_mfma<float>), dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, A, stride, B, stride, ld,
+                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate);
 }
 void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh, const uint16_t* Bl, size_t stride,
                       int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles, int ntile,
                       double* P, double alpha, int accumulate, hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 32);
-    hipLaunchKernelGGL(k_kgemm_bf3, dim3(ntile, nsplit), dim3(256), 0, s, Ah, Al, Bh, Bl, stride, ld, kbeg, kend,
-                       nbatch, tiles, ntile, P, alpha, accumulate);
+    hipLaunchKernelGGL(k_kgemm_bf3, dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, Ah, Al, Bh, Bl, stride, ld, kbeg,
+                       kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate);
 }
 
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s) {

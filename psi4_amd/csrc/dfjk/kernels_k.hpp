@@ -14,14 +14,17 @@ void ht_panel_shape(int ncol, int* nbw, int* npanel, int* ldc);
 
 // X[q][m][i] = sum_n B_{q0+q}[m][n] C[n][i]; C [nbfp][ldc], X [nq][xrows][ldc] with
 // xrows = 128*ceil(nT/4); every row of X is written (rows past nbf are zero).
+// zt: device zero tile of at least TILE*TILE*8 bytes (read for absent blocks).
 void launch_ht2_f64(const TileGeom& g, const double* B, int q0, int nq, const double* C, int ldc, int nbw, int npanel,
-                    double* X, int xrows, hipStream_t s);
+                    double* X, int xrows, const void* zt, hipStream_t s);
 void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const float* C, int ldc, int nbw, int npanel,
-                    float* X, int xrows, hipStream_t s);
+                    float* X, int xrows, const void* zt, hipStream_t s);
+// Row waves per half-transform workgroup: 1 (256 threads) or 2 (512 threads); 0 = default.
+void set_ht_rowwaves(int wm);
 // 3-term BF16: B hi/lo [naux][ntp][32][32], C^T hi/lo [ldc][nbfp], X hi/lo [nq][xrows][ldc].
 void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
                    const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
-                   hipStream_t s);
+                   const void* zt, hipStream_t s);
 
 // P[split][tile] (+)= alpha * sum_b A_b[:, kbeg:kend] B_b[:, kbeg:kend]^T over 128x128 tiles;
 // A_b = A + b*stride, rows of length ld, xrows rows (a multiple of 128).

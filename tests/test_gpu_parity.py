@@ -189,3 +189,40 @@ def test_synthetic_generator_matches_oracle(gpu_available):
     Bo = synthetic_B(nbf, 0, naux, seed)
     assert np.abs(Bg - Bo).max() < 1e-14 * np.abs(Bo).max()
     assert np.abs(Bg - Bg.transpose(0, 2, 1)).max() == 0.0
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_mixed_two_panel_incremental_synthetic(gpu_available, prec):
+    """Stable incremental form at ncol = 2*nocc > 320 (two column panels) on a synthetic tensor:
+    mixed K(dD) vs FP64 K(dD) on identical inputs, error relative to |K(dD)|."""
+    from psi4_amd.core import JK
+    from psi4_amd.workloads import synthetic_C
+    nbf, naux, nocc = 300, 96, 300
+    jk = JK.build_tensor(nbf, naux)
+    jk.set_print(0)
+    jk.initialize()
+    jk.fill_synthetic(99)
+    C = synthetic_C(nbf, nocc, 5)
+    rng = np.random.default_rng(6)
+    Co = np.linalg.qr(C + 1e-4 * rng.standard_normal(C.shape))[0]
+    u, _, vt = np.linalg.svd(Co.T @ C)
+    Co = Co @ (u @ vt)
+    dC = C - Co
+    L, R = np.hstack([C, dC]), np.hstack([dC, Co])
+    out = {}
+    for p in (0, prec):
+        announce(f"two-panel stable incremental, precision {p}")
+        jk.set_precision(p)
+        try:
+            _, J, K = _run(jk, [L], [R])
+        finally:
+            jk.set_precision(0)
+        out[p] = (J[0].copy(), K[0].copy())
+        # full symmetric build at the same precision for scale
+    _, _, Kfull = _run(jk, [C])
+    err = np.abs(out[prec][1] - out[0][1]).max()
+    scale_d = np.abs(out[0][1]).max()
+    print(f"  prec {prec}: |dK| max {scale_d:.3e}  err {err:.3e}  rel {err / scale_d:.3e}  |K| max {np.abs(Kfull[0]).max():.3e}",
+          flush=True)
+    assert np.abs(out[prec][0] - out[0][0]).max() < 1e-11
+    assert err < 5 * REL_K[prec] * scale_d
