@@ -96,10 +96,18 @@ int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const do
 /* Restrict the auxiliary index range used by compute (J and K are additive over Q). */
 int dfjk_set_aux_range(dfjk_handle* h, int q0, int q1);
 
-/* Adjoint of E = tr(D(2J - K)), D = C C^T (no psi4 counterpart; the DF gradient terms of
- * scfgrad/jk_grad.cc:294-832 contract the same quantities):
- *   dEdC = 4 (2J - K) C   (nbf x nocc),   dEdB[Q][m][n] = 4 d_Q D_mn - 2 (D B_Q D)_mn.
- * Device pointers; dEdB may be NULL. */
+/* Adjoint of E = tr(D(2J - K)), D = C C^T, in FP64 whatever the precision setting.  psi4 has
+ * no single counterpart: its DF-SCF gradient (scfgrad/jk_grad.cc:294-475, 634-832) contracts
+ * the same intermediates, c_Q = (Q|mn) D_mn and (Q|ij) = C^T (Q|mn) C.
+ *   E    = sum_mn D_mn (2J - K)_mn                       (host scalar; may be NULL)
+ *   dEdC = 4 (2J - K) C                                  (nbf x nocc; may be NULL)
+ *   dEdB[Q][m][n] = 4 d_Q D_mn - 2 (D B_Q D)_mn          (may be NULL)
+ * E and dEdC use every auxiliary function; dEdB covers Q in the range set by
+ * dfjk_set_aux_range (all by default) and has (q1 - q0) x nbf x nbf elements.  dEdB is the
+ * derivative with respect to each element of the dense symmetric B: a perturbation of the
+ * stored pair (m,n),(n,m) together changes E by dEdB[Q][m][n] + dEdB[Q][n][m].
+ * dfjk_energy_adjoint takes host arrays, the _device variant device arrays (E stays host). */
+int dfjk_energy_adjoint(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB);
 int dfjk_energy_adjoint_device(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB);
 
 /* Timing of the last compute when profiling is on: milliseconds per kernel class and launch counts. */

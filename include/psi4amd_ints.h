@@ -39,6 +39,10 @@ int p4a_ints_nbf(const p4a_basis* b, int* nbf);
 int p4a_ints_overlap(const p4a_basis* b, double* S);
 int p4a_ints_kinetic(const p4a_basis* b, double* T);
 int p4a_ints_potential(const p4a_basis* b, int natom, const double* Z, const double* xyz, double* V);
+/* <mu| r_d - origin_d |nu>, d = x, y, z, into M [3][nbf][nbf]; origin NULL = (0,0,0).
+ * psi4's MintsHelper::ao_dipole (libmints/mintshelper.cc:1627-1645) returns these times the
+ * electron charge, -1. */
+int p4a_ints_dipole(const p4a_basis* b, const double* origin, double* M);
 /* (P|Q) metric, naux x naux (fittingmetric.cc:115-155). */
 int p4a_ints_metric(const p4a_basis* aux, double* J, int nthreads);
 /* (P|mn) written to out[P*p_stride + dst[m*nbf+n]] for every dst >= 0;

@@ -162,7 +162,7 @@ struct dfjk_handle {
     int nT = 0, nbfp = 0, ntp = 0;
     std::vector<int> tidx;
     std::vector<int2> tiles;
-    DevBuf d_tidx, d_tiles;
+    DevBuf d_tidx, d_tiles, d_mask;
     // output 128-tile lists
     int nB = 0;
     DevBuf d_otiles_sym, d_otiles_all, d_otile_of_sym, d_otile_of_all;
@@ -174,6 +174,7 @@ struct dfjk_handle {
     DevBuf dZero;  // one zero TILE x TILE FP64 block, read in place of absent blocks
     // workspace
     DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G, ws_sign, ws_c32, ws_cbf, ws_cmap;
+    DevBuf ws_ct, ws_red, ws_Y, ws_PC, ws_gam, ws_dc;
     // timing
     EventTimer timer;
     double t_ms[4] = {0, 0, 0, 0};
@@ -265,6 +266,9 @@ void upload_tiles(dfjk_handle* h) {
     hcheck(hipMemcpy(h->d_tiles.get<int2>(h->tiles.size()), h->tiles.data(), h->tiles.size() * sizeof(int2),
                      hipMemcpyHostToDevice),
            "upload tiles");
+    hcheck(hipMemcpy(h->d_mask.get<uint8_t>(h->fun_mask.size()), h->fun_mask.data(), h->fun_mask.size(),
+                     hipMemcpyHostToDevice),
+           "upload mask");
     hcheck(hipMemcpy(h->d_otiles_sym.get<int2>(sym.size()), sym.data(), sym.size() * sizeof(int2),
                      hipMemcpyHostToDevice),
            "upload");
@@ -691,6 +695,122 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
     finish_timers(h);
 }
 
+
+// Adjoint of E = tr(D(2J - K)), D = C C^T, at FP64 (the DF gradient terms of
+// scfgrad/jk_grad.cc:294-475 contract the same c_Q = d_Q and (Q|ij) = C^T B_Q C):
+//   dE/dC = 4 (2J - K) C,   Gamma_Qmn = dE/dB_Qmn = 4 d_Q D_mn - 2 (C (C^T X_Q) C^T)_mn, X_Q = B_Q C.
+// E and dE/dC use the whole auxiliary range; Gamma is produced for the configured range
+// [q0, q1), in blocks of Q.  dC is a device [nbf][nocc] matrix.  Gamma blocks go to
+// gam_dev (device, [q1-q0][nbf][nbf]) or, when gam_host is set, are copied to host memory.
+void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double* dEdC, double* gam_dev,
+                  double* gam_host) {
+    const hipStream_t s = h->stream;
+    const int nbf = h->nbf, nbfp = h->nbfp;
+    const size_t nn = static_cast<size_t>(nbf) * nbf;
+    const int uq0 = h->aux_q0, uq1 = h->aux_q1;
+    const int prec = h->precision;
+    const bool dj = h->do_J, dk = h->do_K;
+    double* D = h->ws_D.get<double>(nn);
+    double* Jm = h->ws_J.get<double>(nn);
+    double* Km = h->ws_K.get<double>(nn);
+    // full-range FP64 J and K (this also leaves d_Q for every Q in ws_d)
+    h->aux_q0 = 0;
+    h->aux_q1 = h->naux;
+    h->precision = DFJK_PREC_FP64;
+    h->do_J = h->do_K = true;
+    try {
+        compute_pair(h, dC, nullptr, nocc, D, Jm, Km, 0);
+    } catch (...) {
+        h->aux_q0 = uq0, h->aux_q1 = uq1, h->precision = prec, h->do_J = dj, h->do_K = dk;
+        throw;
+    }
+    h->aux_q0 = uq0, h->aux_q1 = uq1, h->precision = prec, h->do_J = dj, h->do_K = dk;
+    const double* dvec = h->ws_d.as<double>();
+
+    // G = 2J - K; E = <D, G> (fixed-order reduction); dE/dC = 4 G C
+    double* G = h->ws_G.get<double>(nn);
+    launch_axpby_mat(Jm, 2.0, Km, -1.0, G, nn, s);
+    if (E) {
+        constexpr int kRed = 256;
+        double* part = h->ws_red.get<double>(kRed);
+        launch_dot_partial(D, G, nn, part, kRed, s);
+        std::vector<double> hp(kRed);
+        hcheck(hipMemcpyAsync(hp.data(), part, kRed * sizeof(double), hipMemcpyDeviceToHost, s), "energy");
+        hcheck(hipStreamSynchronize(s), "energy");
+        double e = 0.0;
+        for (double v : hp) e += v;
+        *E = e;
+    }
+    if (dEdC) launch_gemm_nn_f64(G, nbf, dC, nocc, dEdC, nocc, nbf, nocc, nbf, 4.0, 0.0, s);
+    if (!gam_dev && !gam_host) return;
+
+    // Gamma over [uq0, uq1) in blocks: X_Q = B_Q C (half-transform), Y_Q = C^T X_Q,
+    // P_Q = C Y_Q, Gamma_Q = 4 d_Q D - 2 P_Q C^T.
+    const TileGeom g = h->geom();
+    const int xrows = (h->nT + 3) / 4 * 128;
+    int nbw, npanel, ldc;
+    ht_panel_shape(nocc, &nbw, &npanel, &ldc);
+    std::vector<int> cmap(ldc, -1);
+    for (int i = 0; i < nocc; ++i) cmap[i] = i;
+    int* dmap = h->ws_cmap.get<int>(ldc);
+    hcheck(hipMemcpy(dmap, cmap.data(), ldc * sizeof(int), hipMemcpyHostToDevice), "column map");
+    double* cl = h->ws_cl.get<double>(static_cast<size_t>(nbfp) * ldc);
+    launch_pack_cmap_f64(dC, nocc, nbf, dmap, cl, nbfp, ldc, s);
+    double* CT = h->ws_ct.get<double>(static_cast<size_t>(nocc) * nbf);
+    launch_transpose(dC, nbf, nocc, nocc, CT, nbf, s);
+
+    const size_t xq = static_cast<size_t>(xrows) * ldc;
+    const size_t yq = static_cast<size_t>(nocc) * nocc, pq = static_cast<size_t>(nbf) * nocc;
+    const size_t per_q = (xq + yq + pq + (gam_host ? nn : 0)) * sizeof(double);
+    size_t budget = size_t(2) << 30;  // bytes of per-Q workspace
+    if (h->memory_doubles) {
+        const size_t resident = (b_doubles(h) + 4 * nn) * sizeof(double);
+        const size_t mem = h->memory_doubles * sizeof(double);
+        budget = mem > resident ? std::min(budget, mem - resident) : 0;
+    }
+    const int nq_total = uq1 - uq0;
+    int qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
+    qblk = std::min(qblk, 65535);
+    double* X = h->ws_XL.get<double>(xq * qblk);
+    double* Y = h->ws_Y.get<double>(yq * qblk);
+    double* P = h->ws_PC.get<double>(pq * qblk);
+    double* Gb = gam_host ? h->ws_gam.get<double>(nn * qblk) : nullptr;
+    for (int qa = uq0; qa < uq1; qa += qblk) {
+        const int nq = std::min(qblk, uq1 - qa);
+        double* out = gam_host ? Gb : gam_dev + static_cast<size_t>(qa - uq0) * nn;
+        launch_ht2_f64(g, h->dB.as<double>(), qa, nq, cl, ldc, nbw, npanel, X, xrows, h->dZero.p, s);
+        launch_gemm_nn_f64(CT, nbf, X, ldc, Y, nocc, nocc, nocc, nbf, 1.0, 0.0, s, nq, 0, xq, yq);
+        launch_gemm_nn_f64(dC, nocc, Y, nocc, P, nocc, nbf, nocc, nocc, 1.0, 0.0, s, nq, 0, yq, pq);
+        launch_fill_scaled(dvec, qa, nq, 4.0, D, nbf, out, s);
+        launch_gemm_nn_f64(P, nocc, CT, nbf, out, nbf, nbf, nbf, nocc, -2.0, 1.0, s, nq, pq, 0, nn);
+        if (gam_host) {
+            hcheck(hipMemcpyAsync(gam_host + static_cast<size_t>(qa - uq0) * nn, Gb, nn * nq * sizeof(double),
+                                  hipMemcpyDeviceToHost, s),
+                   "download dE/dB");
+            hcheck(hipStreamSynchronize(s), "adjoint");
+        }
+    }
+}
+
+void adjoint_entry(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB, bool device) {
+    require_state(h, true);
+    if (nocc <= 0 || !C) fail(DFJK_ERR_ARG, "energy adjoint: C must be nbf x nocc with nocc > 0");
+    const size_t csz = static_cast<size_t>(h->nbf) * nocc;
+    reset_timers(h);
+    if (device) {
+        adjoint_impl(h, C, nocc, E, dEdC, dEdB, nullptr);
+        hcheck(hipStreamSynchronize(h->stream), "adjoint");
+    } else {
+        double* dc = h->ws_craw.get<double>(csz);
+        hcheck(hipMemcpyAsync(dc, C, csz * sizeof(double), hipMemcpyHostToDevice, h->stream), "upload C");
+        double* g = dEdC ? h->ws_dc.get<double>(csz) : nullptr;
+        adjoint_impl(h, dc, nocc, E, g, nullptr, dEdB);
+        if (dEdC) hcheck(hipMemcpyAsync(dEdC, g, csz * sizeof(double), hipMemcpyDeviceToHost, h->stream), "dE/dC");
+        hcheck(hipStreamSynchronize(h->stream), "adjoint");
+    }
+    finish_timers(h);
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -858,33 +978,30 @@ int dfjk_set_B_dense(dfjk_handle* h, const double* B, int on_device) {
     return guarded([&] {
         require_state(h, true);
         if (!B) fail(DFJK_ERR_ARG, "null B");
-        const int n = h->nbf;
-        const size_t nn = static_cast<size_t>(n) * n;
-        std::vector<double> host;
-        const double* src = B;
-        if (on_device) {
-            host.resize(static_cast<size_t>(h->naux) * nn);
-            hcheck(hipMemcpy(host.data(), B, host.size() * sizeof(double), hipMemcpyDeviceToHost), "download B");
-            src = host.data();
-        }
+        const size_t nn = static_cast<size_t>(h->nbf) * h->nbf;
+        if (h->fun_mask.size() != nn) fail(DFJK_ERR_INTERNAL, "screening mask missing");
         invalidate_copies(h);
-        const size_t qs = static_cast<size_t>(h->ntp) * TILE2;
-        std::vector<double> packed(qs);
-        for (int Q = 0; Q < h->naux; ++Q) {
-            std::fill(packed.begin(), packed.end(), 0.0);
-            const double* BQ = src + Q * nn;
-            for (int t = 0; t < h->ntp; ++t) {
-                const int2 IJ = h->tiles[t];
-                for (int r = 0; r < TILE; ++r)
-                    for (int c = 0; c < TILE; ++c) {
-                        const int m = IJ.x * TILE + r, q = IJ.y * TILE + c;
-                        if (m < n && q < n && h->fun_mask[static_cast<size_t>(m) * n + q])
-                            packed[static_cast<size_t>(t) * TILE2 + r * TILE + c] = BQ[static_cast<size_t>(m) * n + q];
-                    }
+        const TileGeom g = h->geom();
+        const hipStream_t s = h->stream;
+        if (on_device) {
+            for (int q0 = 0; q0 < h->naux; q0 += 32768) {
+                const int nq = std::min(h->naux - q0, 32768);
+                launch_pack_dense(g, B + static_cast<size_t>(q0) * nn, h->d_mask.as<uint8_t>(), q0, nq,
+                                  h->dB.as<double>(), s);
             }
-            hcheck(hipMemcpy(h->dB.as<double>() + Q * qs, packed.data(), qs * sizeof(double), hipMemcpyHostToDevice),
-                   "upload B");
+        } else {
+            // stage host Q-slices of about 256 MiB through a device buffer
+            const int qc = static_cast<int>(std::max<size_t>(1, std::min<size_t>(32768, (size_t(32) << 20) / nn)));
+            double* stage = h->ws_XL.get<double>(nn * qc);
+            for (int q0 = 0; q0 < h->naux; q0 += qc) {
+                const int nq = std::min(h->naux - q0, qc);
+                hcheck(hipMemcpyAsync(stage, B + static_cast<size_t>(q0) * nn, nn * nq * sizeof(double),
+                                      hipMemcpyHostToDevice, s),
+                       "upload B");
+                launch_pack_dense(g, stage, h->d_mask.as<uint8_t>(), q0, nq, h->dB.as<double>(), s);
+            }
         }
+        hcheck(hipStreamSynchronize(s), "set B");
     });
 }
 
@@ -943,16 +1060,12 @@ int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const do
     return guarded([&] { compute_impl(h, n, Cl, Cr, ncol, D, J, K, accumulate, true); });
 }
 
+int dfjk_energy_adjoint(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB) {
+    return guarded([&] { adjoint_entry(h, C, nocc, E, dEdC, dEdB, false); });
+}
+
 int dfjk_energy_adjoint_device(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB) {
-    return guarded([&] {
-        (void)C;
-        (void)nocc;
-        (void)E;
-        (void)dEdC;
-        (void)dEdB;
-        require_state(h, true);
-        fail(DFJK_ERR_ARG, "energy adjoint is not available in this build");
-    });
+    return guarded([&] { adjoint_entry(h, C, nocc, E, dEdC, dEdB, true); });
 }
 
 int dfjk_get_timings(dfjk_handle* h, double* ms_j, double* ms_ht, double* ms_kgemm, double* ms_other, int* n_ht,
@@ -982,9 +1095,10 @@ int dfjk_finalize(dfjk_handle* h) {
     return guarded([&] {
         require_state(h, false);
         if (h->initialized && h->stream) hcheck(hipStreamSynchronize(h->stream), "finalize");
-        DevBuf* ws[] = {&h->ws_craw, &h->ws_cl, &h->ws_cr, &h->ws_D, &h->ws_W, &h->ws_dpart, &h->ws_d, &h->ws_Jpart,
-                        &h->ws_XL,   &h->ws_XR, &h->ws_P,  &h->ws_Pd, &h->ws_J, &h->ws_K,     &h->ws_G,
-                        &h->ws_sign, &h->ws_c32, &h->ws_cbf, &h->ws_cmap};
+        DevBuf* ws[] = {&h->ws_craw, &h->ws_cl,  &h->ws_cr,  &h->ws_D,    &h->ws_W,  &h->ws_dpart, &h->ws_d,
+                        &h->ws_Jpart, &h->ws_XL, &h->ws_XR,  &h->ws_P,    &h->ws_Pd, &h->ws_J,     &h->ws_K,
+                        &h->ws_G,    &h->ws_sign, &h->ws_c32, &h->ws_cbf, &h->ws_cmap, &h->ws_ct,   &h->ws_red,
+                        &h->ws_Y,    &h->ws_PC,  &h->ws_gam, &h->ws_dc};
         for (DevBuf* b : ws) b->release();
     });
 }

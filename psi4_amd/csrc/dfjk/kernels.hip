@@ -210,12 +210,17 @@ __global__ void k_tile_reduce(const double* __restrict__ P, int nsplit, int ntil
     *o = alpha * s + (accumulate ? *o : 0.0);
 }
 
-// Row-major C = alpha*A*B + beta*C, 128x128 output tiles, FP64 MFMA.  grid = (ceil(N/128), ceil(M/128)).
+// Row-major C = alpha*A*B + beta*C, 128x128 output tiles, FP64 MFMA.
+// grid = (ceil(N/128), ceil(M/128), batch); operand z is offset by z * stride (0 = shared).
 __global__ __launch_bounds__(256) void k_gemm_nn_f64(const double* __restrict__ A, int lda,
                                                      const double* __restrict__ Bm, int ldb, double* __restrict__ Cm,
-                                                     int ldc, int M, int N, int K, double alpha, double beta) {
+                                                     int ldc, int M, int N, int K, double alpha, double beta,
+                                                     size_t sA, size_t sB, size_t sC) {
     __shared__ double As[128][17];
     __shared__ double Bs[16][132];
+    A += blockIdx.z * sA;
+    Bm += blockIdx.z * sB;
+    Cm += blockIdx.z * sC;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
@@ -367,10 +372,90 @@ void launch_tile_reduce(const double* P, int nsplit, int ntile, const int* d_til
 }
 
 void launch_gemm_nn_f64(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
-                        double alpha, double beta, hipStream_t s) {
-    require(lda >= K && ldb >= N && ldc >= N, "gemm_nn_f64");
-    dim3 grid((N + 127) / 128, (M + 127) / 128);
-    hipLaunchKernelGGL(k_gemm_nn_f64, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, alpha, beta);
+                        double alpha, double beta, hipStream_t s, int batch, size_t sA, size_t sB, size_t sC) {
+    require(lda >= K && ldb >= N && ldc >= N && batch >= 1 && batch <= 65535, "gemm_nn_f64");
+    dim3 grid((N + 127) / 128, (M + 127) / 128, batch);
+    hipLaunchKernelGGL(k_gemm_nn_f64, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, alpha, beta, sA, sB, sC);
+}
+
+// ---- adjoint helpers ----
+__global__ void k_axpby_mat(const double* __restrict__ X, double a, const double* __restrict__ Y, double b,
+                            double* __restrict__ Z, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) Z[i] = a * X[i] + b * Y[i];
+}
+// partial[b] = sum over the block's slice of X.*Y (fixed tree, deterministic)
+__global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ X, const double* __restrict__ Y,
+                                                     size_t n, double* __restrict__ partial) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256)
+        s = fma(X[i], Y[i], s);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+__global__ void k_transpose(const double* __restrict__ A, int rows, int cols, int lda, double* __restrict__ AT,
+                            int ldt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (c < cols) AT[static_cast<size_t>(c) * ldt + r] = A[static_cast<size_t>(r) * lda + c];
+}
+// G[q][m][n] = a * d[q0+q] * D[m][n]
+__global__ void k_fill_scaled(const double* __restrict__ d, int q0, double a, const double* __restrict__ D, int n,
+                              double* __restrict__ G) {
+    const size_t nn = static_cast<size_t>(n) * n;
+    const int q = blockIdx.y;
+    const double f = a * d[q0 + q];
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < nn;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        G[q * nn + i] = f * D[i];
+}
+
+void launch_axpby_mat(const double* X, double a, const double* Y, double b, double* Z, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpby_mat, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, X, a, Y, b, Z, n);
+}
+void launch_dot_partial(const double* X, const double* Y, size_t n, double* partial, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(nblocks), dim3(256), 0, s, X, Y, n, partial);
+}
+void launch_transpose(const double* A, int rows, int cols, int lda, double* AT, int ldt, hipStream_t s) {
+    require(lda >= cols && ldt >= rows, "transpose");
+    hipLaunchKernelGGL(k_transpose, dim3((cols + 255) / 256, rows), dim3(256), 0, s, A, rows, cols, lda, AT, ldt);
+}
+void launch_fill_scaled(const double* d, int q0, int nq, double a, const double* D, int n, double* G, hipStream_t s) {
+    require(nq >= 1 && nq <= 65535, "fill_scaled");
+    hipLaunchKernelGGL(k_fill_scaled, dim3(64, nq), dim3(256), 0, s, d, q0, a, D, n, G);
+}
+
+// Dense [nq][nbf][nbf] -> stored tiles of Q = q0 .. q0+nq-1; pairs outside the Schwarz mask are zero.
+__global__ __launch_bounds__(256) void k_pack_dense(const double* __restrict__ src, const int2* __restrict__ tiles,
+                                                    int ntp, int nbf, const uint8_t* __restrict__ mask, int q0,
+                                                    double* __restrict__ B) {
+    const int t = blockIdx.x, q = blockIdx.y;
+    const int2 IJ = tiles[t];
+    const double* S = src + static_cast<size_t>(q) * nbf * nbf;
+    double* o = B + (static_cast<size_t>(q0 + q) * ntp + t) * TILE2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int e = threadIdx.x + 256 * k;
+        const int r = e >> 5, c = e & 31;
+        const int m = IJ.x * TILE + r, n = IJ.y * TILE + c;
+        double v = 0.0;
+        if (m < nbf && n < nbf) {
+            const size_t mn = static_cast<size_t>(m) * nbf + n;
+            if (mask[mn]) v = S[mn];
+        }
+        o[e] = v;
+    }
+}
+
+void launch_pack_dense(const TileGeom& g, const double* src, const uint8_t* mask, int q0, int nq, double* B,
+                       hipStream_t s) {
+    require(q0 >= 0 && nq >= 1 && nq <= 65535, "pack_dense");
+    hipLaunchKernelGGL(k_pack_dense, dim3(g.ntp, nq), dim3(256), 0, s, src, g.d_tiles, g.ntp, g.nbf, mask, q0, B);
 }
 
 void launch_synth_b(const TileGeom& g, const double* decay, uint64_t seed, int q0, int q1, double* B, hipStream_t s) {

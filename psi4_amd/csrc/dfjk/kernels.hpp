@@ -52,11 +52,26 @@ void launch_tile_reduce(const double* P, int nsplit, int ntile, const int* d_til
                         double* out, int ldo, double alpha, int accumulate, hipStream_t s);
 
 // Plain FP64 GEMM, row-major, C = alpha*A*B + beta*C (A MxK lda, B KxN ldb).  MFMA, 128x128 tiles.
+// batch > 1: operand z at A + z*sA, B + z*sB, C + z*sC (a zero stride shares the operand).
 void launch_gemm_nn_f64(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
-                        double alpha, double beta, hipStream_t s);
+                        double alpha, double beta, hipStream_t s, int batch = 1, size_t sA = 0, size_t sB = 0,
+                        size_t sC = 0);
+
+// ---- adjoint helpers ----
+// Z = a X + b Y (n elements)
+void launch_axpby_mat(const double* X, double a, const double* Y, double b, double* Z, size_t n, hipStream_t s);
+// partial[0..nblocks) = block sums of X.*Y (deterministic; sum the partials on the host)
+void launch_dot_partial(const double* X, const double* Y, size_t n, double* partial, int nblocks, hipStream_t s);
+// AT[c][r] = A[r][c]
+void launch_transpose(const double* A, int rows, int cols, int lda, double* AT, int ldt, hipStream_t s);
+// G[q][:][:] = a * d[q0+q] * D for q < nq (D n x n)
+void launch_fill_scaled(const double* d, int q0, int nq, double a, const double* D, int n, double* G, hipStream_t s);
 
 // ---- synthetic tensor-level workload ----
 // B_Qmn = s_Q * decay[|m-n|] * (G_Qmn + G_Qnm)/2, G from Philox4x32-10 keyed by seed, Q in [q0,q1).
+// Stored tiles of Q = q0 .. q0+nq-1 from a dense device [nq][nbf][nbf] tensor; mask is nbf x nbf (0 = screened).
+void launch_pack_dense(const TileGeom& g, const double* src, const uint8_t* mask, int q0, int nq, double* B,
+                       hipStream_t s);
 void launch_synth_b(const TileGeom& g, const double* decay, uint64_t seed, int q0, int q1, double* B, hipStream_t s);
 
 }  // namespace p4a

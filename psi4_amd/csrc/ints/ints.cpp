@@ -570,9 +570,52 @@ void one_electron(const Basis& b, OneE kind, int natom, const double* Z, const d
     }
 }
 
+// Dipole integrals <a| r_d - O_d |b>, d = x, y, z, into M[3][nbf][nbf].  With the
+// Hermite expansion of the pair overlap distribution, x_C = X_PC + (x - X_P) and
+// integral (x - X_P) Lambda_t = delta_t1 (pi/p)^1/2, so
+// <a|x_C|b>_x = (E^{ij}_1 + X_PC E^{ij}_0) (pi/p)^1/2 (Helgaker, Jorgensen, Olsen 9.5.41).
+void dipole_ints(const Basis& b, const double* O, double* M) {
+    const int ns = static_cast<int>(b.shells.size());
+    const size_t nn = static_cast<size_t>(b.nbf) * b.nbf;
+#pragma omp parallel for schedule(dynamic)
+    for (int I = 0; I < ns; ++I) {
+        for (int J = 0; J <= I; ++J) {
+            const Shell& A = b.shells[I];
+            const Shell& B = b.shells[J];
+            PairData pd = make_pair(A, B);
+            const auto& ca = carts().comps[A.l];
+            const auto& cb = carts().comps[B.l];
+            std::vector<double> blk[3];
+            for (auto& v : blk) v.assign(ca.size() * cb.size(), 0.0);
+            const int st = pd.lab + 1;
+            for (size_t k = 0; k < pd.prims.size(); ++k) {
+                const PrimPair& pp = pd.prims[k];
+                const double* E = pd.Eprim(k);
+                const double s0 = std::pow(kPi / pp.p, 1.5) * pp.pref;
+                size_t kk = 0;
+                for (auto& a : ca)
+                    for (auto& bb : cb) {
+                        double e0[3], e1[3];
+                        for (int d = 0; d < 3; ++d) {
+                            const double* e = E + d * pd.esz + (a[d] * (pd.lb + 1) + bb[d]) * st;
+                            e0[d] = e[0];
+                            e1[d] = (st > 1 ? e[1] : 0.0) + (pp.P[d] - O[d]) * e[0];
+                        }
+                        blk[0][kk] += s0 * e1[0] * e0[1] * e0[2];
+                        blk[1][kk] += s0 * e0[0] * e1[1] * e0[2];
+                        blk[2][kk] += s0 * e0[0] * e0[1] * e1[2];
+                        ++kk;
+                    }
+            }
+            for (int d = 0; d < 3; ++d) scatter_1e(b, I, J, blk[d], M + d * nn);
+        }
+    }
+}
+
 }  // namespace
 
 void overlap(const Basis& b, double* S) { one_electron(b, OneE::S, 0, nullptr, nullptr, S); }
+void dipole(const Basis& b, const double* origin, double* M) { dipole_ints(b, origin, M); }
 void kinetic(const Basis& b, double* T) {
     one_electron(b, OneE::T, 0, nullptr, nullptr, T);
 }

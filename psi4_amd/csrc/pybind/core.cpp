@@ -107,7 +107,11 @@ class JK {
     void set_precision(int p) { check(dfjk_set_precision(h_, p)); precision_ = p; }
     void set_fitting(int f) { check(dfjk_set_fitting(h_, f)); }
     void set_profiling(bool on) { check(dfjk_set_profiling(h_, on)); }
-    void set_aux_range(int q0, int q1) { check(dfjk_set_aux_range(h_, q0, q1)); }
+    void set_aux_range(int q0, int q1) {
+        check(dfjk_set_aux_range(h_, q0, q1));
+        aux_q0_ = q0;
+        aux_q1_ = q1;
+    }
     void set_accumulate(bool a) { accumulate_ = a; }
 
     py::list& C_left() { return Cl_; }
@@ -199,6 +203,38 @@ class JK {
     void set_B_dense(const Arr& B) {
         check(dfjk_set_B_dense(h_, B.data(), 0));
     }
+    void set_B_dense_device(uintptr_t B) { check(dfjk_set_B_dense(h_, reinterpret_cast<const double*>(B), 1)); }
+
+    // Adjoint of E = tr(D(2J-K)), D = C C^T: returns (E, dE/dC, dE/dB or None).
+    py::tuple energy_adjoint(const Arr& C, bool want_dEdB) {
+        if (C.ndim() != 2 || C.shape(0) != nbf_) throw std::runtime_error("JK: C must be nbf x nocc");
+        const int nocc = static_cast<int>(C.shape(1));
+        int naux = 0;
+        check(dfjk_get_info(h_, nullptr, &naux, nullptr, nullptr));
+        const int q0 = aux_q1_ > aux_q0_ ? aux_q0_ : 0, q1 = aux_q1_ > aux_q0_ ? aux_q1_ : naux;
+        Arr dC({static_cast<py::ssize_t>(nbf_), static_cast<py::ssize_t>(nocc)});
+        py::object dB = py::none();
+        double* pB = nullptr;
+        if (want_dEdB) {
+            Arr b({static_cast<py::ssize_t>(q1 - q0), static_cast<py::ssize_t>(nbf_), static_cast<py::ssize_t>(nbf_)});
+            pB = b.mutable_data();
+            dB = b;
+        }
+        double E = 0.0;
+        double* pC = dC.mutable_data();
+        {
+            py::gil_scoped_release nogil;
+            check(dfjk_energy_adjoint(h_, C.data(), nocc, &E, pC, pB));
+        }
+        return py::make_tuple(E, dC, dB);
+    }
+    double energy_adjoint_device(uintptr_t C, int nocc, uintptr_t dEdC, uintptr_t dEdB) {
+        double E = 0.0;
+        py::gil_scoped_release nogil;
+        check(dfjk_energy_adjoint_device(h_, reinterpret_cast<const double*>(C), nocc, &E,
+                                         reinterpret_cast<double*>(dEdC), reinterpret_cast<double*>(dEdB)));
+        return E;
+    }
     void fill_synthetic(uint64_t seed) { check(dfjk_fill_synthetic(h_, seed)); }
     Arr get_B_dense(int q0, int q1) {
         Arr out({static_cast<py::ssize_t>(q1 - q0), static_cast<py::ssize_t>(nbf_), static_cast<py::ssize_t>(nbf_)});
@@ -274,6 +310,7 @@ class JK {
     }
     dfjk_handle* h_;
     int nbf_;
+    int aux_q0_ = 0, aux_q1_ = 0;
     std::string name_;
     py::list Cl_, Cr_;
     std::vector<Arr> D_, J_, K_;
@@ -336,6 +373,10 @@ PYBIND11_MODULE(_core, m) {
         .def("wK", &JK::wK)
         .def("D", &JK::D)
         .def("set_B_dense", &JK::set_B_dense)
+        .def("set_B_dense_device", &JK::set_B_dense_device)
+        .def("energy_adjoint", &JK::energy_adjoint, py::arg("C"), py::arg("want_dEdB") = false)
+        .def("energy_adjoint_device", &JK::energy_adjoint_device, py::arg("C"), py::arg("nocc"), py::arg("dEdC"),
+             py::arg("dEdB"))
         .def("fill_synthetic", &JK::fill_synthetic)
         .def("get_B_dense", &JK::get_B_dense)
         .def("timings", &JK::timings)

@@ -42,6 +42,17 @@ def kinetic(basis):
     return T
 
 
+def dipole(basis, origin=(0.0, 0.0, 0.0)):
+    """<mu| r - origin |nu> as [3, nbf, nbf] (x, y, z; no electron charge)."""
+    lib = load_ints_lib()
+    n = basis.nbf()
+    M = np.zeros((3, n, n))
+    cb, _keep = basis.as_c()
+    o = np.ascontiguousarray(origin, np.float64)
+    _check(lib, lib.p4a_ints_dipole(ctypes.byref(cb), _ptr(o), _ptr(M)))
+    return M
+
+
 def potential(basis, mol):
     lib = load_ints_lib()
     n = basis.nbf()

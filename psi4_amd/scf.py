@@ -80,7 +80,7 @@ class RHF:
 
     def __init__(self, mol, basis="cc-pvdz", df_basis_scf=None, jk=None, e_convergence=1e-10, d_convergence=1e-8,
                  maxiter=100, diis_max_vecs=10, mixed_precision=None, mixed_until=1e-5, incfock_full_every=5,
-                 fail_on_maxiter=True, jk_callback=None):
+                 fail_on_maxiter=True, jk_callback=None, perturb_dipole=None):
         from .basisset import BasisSet
         self.mol = mol
         self.basis = BasisSet.build(mol, basis)
@@ -99,6 +99,10 @@ class RHF:
         self.jk_callback = jk_callback  # called as cb(iteration, C_occ, J, K, kind) after every build
         self._jk = jk
         self._injected = jk is not None
+        self.Ca = self.Fa = self.eps = None
+        # PERTURB_H / PERTURB_WITH DIPOLE (libmints/mintshelper.cc:1433-1480): V += sum_d lambda_d mu_d
+        # with psi4's electronic dipole integrals mu_d = -<r_d>
+        self.perturb_dipole = None if perturb_dipole is None else np.asarray(perturb_dipole, np.float64)
         if jk is not None and jk.info()["nbf"] != self.basis.nbf():        # hf.cc:400-409
             raise ValueError("JK object basis does not match the wavefunction basis")
 
@@ -109,6 +113,23 @@ class RHF:
             self._jk.set_print(0)
             self._jk.initialize()
         return self._jk
+
+    # CPHF products of the converged reference (rhf.cc:372-842), restated in response.py
+    def onel_Hx(self, x_vec):
+        from . import response
+        return response.onel_Hx(self, x_vec)
+
+    def twoel_Hx_full(self, x_vec, combine=True, return_basis="MO", singlet=True):
+        from . import response
+        return response.twoel_Hx_full(self, x_vec, combine, return_basis, singlet)
+
+    def cphf_Hx(self, x_vec):
+        from . import response
+        return response.cphf_Hx(self, x_vec)
+
+    def cphf_solve(self, x_vec, conv_tol=1e-5, max_iter=10, print_lvl=0):
+        from . import response
+        return response.cphf_solve(self, x_vec, conv_tol, max_iter, print_lvl)
 
     def _full_build(self, Cocc):
         jk = self.jk()
@@ -137,6 +158,8 @@ class RHF:
         mol, bs = self.mol, self.basis
         S = ints.overlap(bs)
         H = ints.kinetic(bs) + ints.potential(bs, mol)
+        if self.perturb_dipole is not None:
+            H = H - np.einsum("d,dmn->mn", self.perturb_dipole, ints.dipole(bs))
         enuc = mol.nuclear_repulsion_energy()
         w, U = np.linalg.eigh(S)
         X = (U * w ** -0.5) @ U.T
@@ -175,6 +198,7 @@ class RHF:
             history.append(dict(iteration=it, energy=E, d_rms=d_rms, build=kind))
             converged = abs(E - E_old) < self.e_conv and d_rms < self.d_conv
             if converged and kind == "fp64":
+                self.Ca, self.Fa, self.eps = C, F, eps
                 return SCFResult(E, C, eps, D, J, K, F, it, history,
                                  dict(setup_s=t_setup, jk_s=t_jk, total_s=time.perf_counter() - t0))
             if mixed_on and d_rms < self.mixed_until:

@@ -25,3 +25,48 @@ def gpu_jk(prim, aux, **kw):
 
 def maxabs(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
+
+
+class OracleJK:
+    """The oracle MemDFJK behind the JK-object methods the RHF and CPHF drivers call
+    (C_clear / C_left_add / C_right_add / compute / J / K / D), for CPU runs of the
+    drivers and as the reference side of GPU driver tests."""
+
+    def __init__(self, prim, aux, **kw):
+        from oracle.dfhelper import MemDFJKOracle
+        self.ora = MemDFJKOracle(prim, aux, **kw)
+        self._L, self._R = [], []
+        self._D = self._J = self._K = []
+
+    def info(self):
+        return {"nbf": self.ora.nbf, "naux": self.ora.naux}
+
+    def set_print(self, _):
+        pass
+
+    def set_precision(self, _):
+        pass
+
+    def initialize(self):
+        pass
+
+    def C_clear(self):
+        self._L, self._R = [], []
+
+    def C_left_add(self, C):
+        self._L.append(np.asarray(C))
+
+    def C_right_add(self, C):
+        self._R.append(np.asarray(C))
+
+    def compute(self):
+        self._D, self._J, self._K = self.ora.compute(self._L, self._R or None)
+
+    def D(self):
+        return self._D
+
+    def J(self):
+        return self._J
+
+    def K(self):
+        return self._K

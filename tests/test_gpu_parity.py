@@ -191,13 +191,14 @@ def test_synthetic_generator_matches_oracle(gpu_available):
     assert np.abs(Bg - Bg.transpose(0, 2, 1)).max() == 0.0
 
 
+@pytest.mark.parametrize("shape", [(420, 96, 200), (1500, 128, 300)])
 @pytest.mark.parametrize("prec", [1, 2])
-def test_mixed_two_panel_incremental_synthetic(gpu_available, prec):
+def test_mixed_two_panel_incremental_synthetic(gpu_available, prec, shape):
     """Stable incremental form at ncol = 2*nocc > 320 (two column panels) on a synthetic tensor:
     mixed K(dD) vs FP64 K(dD) on identical inputs, error relative to |K(dD)|."""
     from psi4_amd.core import JK
     from psi4_amd.workloads import synthetic_C
-    nbf, naux, nocc = 300, 96, 300
+    nbf, naux, nocc = shape
     jk = JK.build_tensor(nbf, naux)
     jk.set_print(0)
     jk.initialize()
@@ -222,7 +223,10 @@ def test_mixed_two_panel_incremental_synthetic(gpu_available, prec):
     _, _, Kfull = _run(jk, [C])
     err = np.abs(out[prec][1] - out[0][1]).max()
     scale_d = np.abs(out[0][1]).max()
-    print(f"  prec {prec}: |dK| max {scale_d:.3e}  err {err:.3e}  rel {err / scale_d:.3e}  |K| max {np.abs(Kfull[0]).max():.3e}",
-          flush=True)
+    D = C @ C.T
+    dE = abs(np.vdot(D, out[prec][1] - out[0][1]))
+    print(f"  prec {prec} shape {shape}: |dK| max {scale_d:.3e}  err {err:.3e}  rel {err / scale_d:.3e}  "
+          f"|K| max {np.abs(Kfull[0]).max():.3e}  |tr(D dK_err)| {dE:.3e}", flush=True)
+    assert scale_d > 1e-6 * np.abs(Kfull[0]).max()
     assert np.abs(out[prec][0] - out[0][0]).max() < 1e-11
     assert err < 5 * REL_K[prec] * scale_d

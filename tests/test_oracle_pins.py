@@ -205,6 +205,46 @@ def test_energy_invariant_under_rotation(h2o2):
     assert abs(e[0] - e[1]) < 1e-12 * abs(e[0])
 
 
+def test_adjoint_energy_and_gradients(h2o2):
+    """oracle.adjoint: E against the dense four-index contraction and the pinned J/K path;
+    dE/dC and dE/dB against central differences of E (E is exactly quadratic in B,
+    quartic in C), single elements and symmetric pairs."""
+    from oracle.adjoint import energy, energy_adjoint
+    mol, prim, aux, o, B, eri = h2o2
+    rng = np.random.default_rng(11)
+    C = rng.standard_normal((prim.nbf(), 6)) * 0.3
+    E, dEdC, Gam = energy_adjoint(B, C)
+    Dd = C @ C.T
+    E4 = 2 * np.einsum("mn,mnls,ls->", Dd, eri, Dd) - np.einsum("mn,mlns,ls->", Dd, eri, Dd)
+    Do, Jo, Ko = o.compute([C])
+    assert abs(E - E4) < 1e-11 * abs(E4)
+    assert abs(E - np.vdot(Do[0], 2 * Jo[0] - Ko[0])) < 1e-11 * abs(E)
+    assert abs(energy(B, C) - E) < 1e-11 * abs(E)
+    h = 1e-5
+    for _ in range(12):
+        m, i = rng.integers(prim.nbf()), rng.integers(6)
+        Cp, Cm = C.copy(), C.copy()
+        Cp[m, i] += h
+        Cm[m, i] -= h
+        fd = (energy(B, Cp) - energy(B, Cm)) / (2 * h)
+        assert abs(fd - dEdC[m, i]) < 1e-6 * max(1.0, abs(dEdC[m, i]))
+    for _ in range(12):
+        q, m, n = rng.integers(B.shape[0]), rng.integers(prim.nbf()), rng.integers(prim.nbf())
+        Bp, Bm = B.copy(), B.copy()
+        Bp[q, m, n] += h
+        Bm[q, m, n] -= h
+        fd = (energy(Bp, C) - energy(Bm, C)) / (2 * h)
+        assert abs(fd - Gam[q, m, n]) < 1e-6 * max(1.0, abs(Gam[q, m, n]))
+        if m != n:  # the stored (packed) element moves both halves
+            Bp[q, n, m] += h
+            Bm[q, n, m] -= h
+            fd = (energy(Bp, C) - energy(Bm, C)) / (2 * h)
+            assert abs(fd - (Gam[q, m, n] + Gam[q, n, m])) < 1e-6 * max(1.0, abs(Gam[q, m, n]))
+    assert np.abs(Gam - Gam.transpose(0, 2, 1)).max() < 1e-12
+    _, _, Gs = energy_adjoint(B, C, 5, 17)
+    assert np.array_equal(Gs, Gam[5:17])
+
+
 def test_cholesky_fit_equals_eigen_fit(h2o2):
     """With no eigenvalue truncated, B' = L^-1 (P|mn) gives the same J, K as J^-1/2 (P|mn)."""
     mol, prim, aux, o, B, eri = h2o2
