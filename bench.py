@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16x3", help="headline (mixed) precision: fp32|bf16x3|fp64")
     ap.add_argument("--no-incremental", action="store_true", help="skip the incremental (dD) build measurement")
-    ap.add_argument("--cpu-sample", type=int, default=200, help="aux functions in the CPU oracle slice (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=1800, help="aux functions in the CPU oracle slice (0: skip)")
     ap.add_argument("--nbf", type=int, default=None)
     ap.add_argument("--naux", type=int, default=None)
     ap.add_argument("--nocc", type=int, default=None)
@@ -55,8 +55,9 @@ def log(msg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding over the aux index is a design "
-                                   "(see DESIGN.md), not a measured path", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "only one GPU is measured: J and K are additive over the auxiliary index, so "
+                                   "several GPUs would each hold a Q-slice of B and sum J/K, which is not "
+                                   "built or measured here", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numpy as np
@@ -89,7 +90,9 @@ def main():
     log(f"synthetic workload nbf={nbf} naux={naux} nocc={nocc}: generating B on the device")
     jk = JK.build_tensor(nbf, naux)
     jk.set_print(0)
-    stream = torch.cuda.current_stream(dev)
+    # one stream for torch's work and the library's kernels (default priority)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     jk.set_stream(stream.cuda_stream)
     jk.initialize()
     jk.fill_synthetic(SYNTHETIC["seed_B"])
@@ -130,46 +133,60 @@ def main():
     jk.set_profiling(False)
 
     # Incremental build (the way the mixed path is used inside SCF): J(dD), K(dD) added to
-    # FP64 J/K of D_old, in the stable form dD = C_new dC^T + dC C_old^T with dC = C_new - C_old
-    # after aligning C_old to C_new (orthogonal Procrustes), i.e. C_left=[C_new, dC],
-    # C_right=[dC, C_old]: every low-precision product carries an O(|dC|) factor.
+    # FP64 J/K of D_old, in the cross form dD = A V^T + V A^T with A = (C_new + C_old)/2,
+    # V = C_new - C_old after aligning C_old to C_new (orthogonal Procrustes), i.e.
+    # C_left=[A, V], C_right=[V, A]: every low-precision product carries the small factor V,
+    # and the engine contracts it as a symmetric K.  The same build in FP64 checks the
+    # identity K(D_old) + K(dD) = K(D_new) at this size.
     inc = None
     if not args.no_incremental:
         rng = np.random.default_rng(SYNTHETIC["seed_C"] + 1)
         Cold_h = np.linalg.qr(C_h + 1e-4 * rng.standard_normal(C_h.shape))[0]
         u, _, vt = np.linalg.svd(Cold_h.T @ C_h)
         Cold_h = Cold_h @ (u @ vt)
-        dC_h = C_h - Cold_h
+        V_h = C_h - Cold_h
+        A_h = 0.5 * (C_h + Cold_h)
         Cold = torch.from_numpy(np.ascontiguousarray(Cold_h)).to(dev)
         Jold = torch.empty_like(D)
         Kold = torch.empty_like(D)
         jk.set_precision(PREC_FP64)
         jk.compute_device([Cold.data_ptr()], [], [nocc], [], [Jold.data_ptr()], [Kold.data_ptr()], False)
-        Cl = torch.from_numpy(np.ascontiguousarray(np.hstack([C_h, dC_h]))).to(dev)
-        Cr = torch.from_numpy(np.ascontiguousarray(np.hstack([dC_h, Cold_h]))).to(dev)
+        Cl = torch.from_numpy(np.ascontiguousarray(np.hstack([A_h, V_h]))).to(dev)
+        Cr = torch.from_numpy(np.ascontiguousarray(np.hstack([V_h, A_h]))).to(dev)
         Ji = torch.empty_like(D)
         Ki = torch.empty_like(D)
+
+        def inc_build():
+            Ji.copy_(Jold)
+            Ki.copy_(Kold)
+            jk.compute_device([Cl.data_ptr()], [Cr.data_ptr()], [2 * nocc], [], [Ji.data_ptr()], [Ki.data_ptr()], True)
+
+        E_inc = {}
+        if "fp64" in results:
+            jk.set_precision(PREC_FP64)
+            inc_build()
+            E_inc["fp64"] = float(torch.sum(D * (2.0 * Ji - Ki)).item())
         jk.set_precision(prec_id[headline])
         log(f"incremental {headline}: {args.warmup} warmup + {args.steps} timed dD builds (ncol={2 * nocc})")
         for _ in range(args.warmup):
-            Ji.copy_(Jold)
-            Ki.copy_(Kold)
-            jk.compute_device([Cl.data_ptr()], [Cr.data_ptr()], [2 * nocc], [], [Ji.data_ptr()], [Ki.data_ptr()], True)
+            inc_build()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            Ji.copy_(Jold)
-            Ki.copy_(Kold)
-            jk.compute_device([Cl.data_ptr()], [Cr.data_ptr()], [2 * nocc], [], [Ji.data_ptr()], [Ki.data_ptr()], True)
+            inc_build()
         torch.cuda.synchronize()
         dt_inc = (time.perf_counter() - t0) / args.steps
-        E_inc = float(torch.sum(D * (2.0 * Ji - Ki)).item()) if "fp64" in results else None
+        E_inc[headline] = float(torch.sum(D * (2.0 * Ji - Ki)).item())
         dD = float(np.abs(C_h @ C_h.T - Cold_h @ Cold_h.T).max())
+        ef = results["fp64"]["E"] if "fp64" in results else None
         inc = {"builds_per_s": round(1.0 / dt_inc, 4), "ms_per_build": round(dt_inc * 1e3, 3),
-               "precision": headline, "form": "C_left=[C_new,dC], C_right=[dC,C_old]", "max_abs_dD": dD,
-               "max_abs_dC": float(np.abs(dC_h).max()),
-               "dE_vs_fp64": (abs(E_inc - results["fp64"]["E"]) if E_inc is not None else None)}
-        log(f"incremental {headline}: {dt_inc * 1e3:.2f} ms/build, |dD|max={dD:.2e}, dE={inc['dE_vs_fp64']}")
+               "speedup_vs_fp64_full": (round(results["fp64"]["s_per_build"] / dt_inc, 3) if ef is not None else None),
+               "precision": headline, "form": "C_left=[A,V], C_right=[V,A]; A=(C_new+C_old)/2, V=C_new-C_old",
+               "max_abs_dD": dD, "max_abs_V": float(np.abs(V_h).max()),
+               "dE_vs_fp64": (abs(E_inc[headline] - ef) if ef is not None else None),
+               "dE_fp64_incremental_vs_fp64_full": (abs(E_inc["fp64"] - ef) if ef is not None else None)}
+        log(f"incremental {headline}: {dt_inc * 1e3:.2f} ms/build, |dD|max={dD:.2e}, dE={inc['dE_vs_fp64']}, "
+            f"FP64 incremental dE={inc['dE_fp64_incremental_vs_fp64_full']}")
         jk.set_precision(prec_id[headline])
 
     # roofline of the dominant kernel class of the headline build
@@ -196,24 +213,42 @@ def main():
     j_roof = {"bound": "hbm", "achieved_GBs": round(j_bytes / (j_ms * 1e-3) / 1e9, 1), "peak_GBs": HBM_PEAK_GBS,
               "bytes_per_build": j_bytes, "ms_per_build": round(j_ms, 4)}
 
-    # CPU oracle on a bounded Q-slice of the same workload
+    # measured HBM traffic of that kernel (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated
+    # against a copy kernel), recorded per dispatch in profiles/ for this workload
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         f"traffic_synthetic_nbf{nbf}_naux{naux}_nocc{nocc}.json")
+    kname = roofline["kernel"].split(" ")[0]
+    if os.path.exists(tfile):
+        rec = json.load(open(tfile)).get("kernels", {}).get(kname)
+        if rec:
+            per_q = (rec["read_bytes"] + rec["write_bytes"]) / rec["q_per_dispatch"]
+            q_per_launch = naux * args.steps / max(1, nlaunch)
+            roofline["traffic"] = round(per_q * q_per_launch)
+            roofline["traffic_source"] = f"{os.path.basename(tfile)} ({rec.get('profiled', 'rocprofv3 --pmc')})"
+
+    # CPU oracle on a bounded Q-slice of the same workload (J and K are additive over Q:
+    # the slice is processed in chunks of 200 aux functions to bound host memory)
     cpu = None
     if args.cpu_sample > 0:
         qs = min(args.cpu_sample, naux)
         log(f"CPU oracle on a Q-slice of {qs} aux functions")
         from oracle.dfhelper import compute_J_symm, compute_K, prepare_sparsity
-        Bq = jk.get_B_dense(0, qs)
-        fm = np.ones((nbf, nbf))
-        sp = prepare_sparsity(fm, np.ones((1, 1)), 1.0, 1.0, qs)
-        Ppq = np.ascontiguousarray(Bq.transpose(1, 0, 2)).reshape(-1)
-        del Bq
         Dh = C_h @ C_h.T
         Jc = np.zeros((nbf, nbf))
         Kc = np.zeros((nbf, nbf))
-        t0 = time.perf_counter()
-        compute_J_symm(Ppq, sp, Dh, Jc)
-        compute_K(Ppq, sp, C_h, C_h, Kc, True)
-        tcpu = time.perf_counter() - t0
+        tcpu = 0.0
+        for qa in range(0, qs, 200):
+            qb = min(qs, qa + 200)
+            Bq = jk.get_B_dense(qa, qb)
+            sp = prepare_sparsity(np.ones((nbf, nbf)), np.ones((1, 1)), 1.0, 1.0, qb - qa)
+            Ppq = np.ascontiguousarray(Bq.transpose(1, 0, 2)).reshape(-1)
+            del Bq
+            t0 = time.perf_counter()
+            compute_J_symm(Ppq, sp, Dh, Jc)
+            compute_K(Ppq, sp, C_h, C_h, Kc, True)
+            tcpu += time.perf_counter() - t0
+            del Ppq
+        log(f"CPU oracle: {tcpu:.2f} s of J/K work on {qs} aux functions")
         try:
             from threadpoolctl import threadpool_info
             cores = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])

@@ -16,7 +16,12 @@
  *   - Host-pointer entry points copy through device buffers owned by the
  *     handle; *_device entry points take device pointers and never copy to the host.
  *   - Calls on one handle are not reentrant; separate handles are independent.
- *   - All device work of a handle runs on one HIP stream (dfjk_set_stream).
+ *   - All device work of a handle runs on one HIP stream (dfjk_set_stream).  The
+ *     handle's own stream is a blocking stream, so it is ordered with work on the
+ *     legacy default (null) stream; a caller on a non-blocking stream passes that
+ *     stream.  *_device calls return with their work queued (host-pointer calls
+ *     return after it finished), except the signs test of a non-symmetric pair and
+ *     the adjoint, which synchronise the stream.
  */
 #ifndef PSI4AMD_DFJK_H
 #define PSI4AMD_DFJK_H

@@ -43,6 +43,9 @@ int p4a_ints_potential(const p4a_basis* b, int natom, const double* Z, const dou
  * psi4's MintsHelper::ao_dipole (libmints/mintshelper.cc:1627-1645) returns these times the
  * electron charge, -1. */
 int p4a_ints_dipole(const p4a_basis* b, const double* origin, double* M);
+/* <mu| (r - origin) x nabla |nu>, x, y, z, into M [3][nbf][nbf] (antisymmetric, no factor -i);
+ * the quantity of MintsHelper::ao_angular_momentum (libmints/mintshelper.cc:1613-1625). */
+int p4a_ints_angular_momentum(const p4a_basis* b, const double* origin, double* M);
 /* (P|Q) metric, naux x naux (fittingmetric.cc:115-155). */
 int p4a_ints_metric(const p4a_basis* aux, double* J, int nthreads);
 /* (P|mn) written to out[P*p_stride + dst[m*nbf+n]] for every dst >= 0;

@@ -201,8 +201,10 @@ void require_state(dfjk_handle* h, bool init_needed) {
 
 // The handle's own stream is created on first device use, so that creating a
 // handle, setting knobs and asking for the memory estimate need no device.
+// The library's own stream is a blocking stream: work a caller queues on the legacy
+// default (null) stream -- torch's default stream -- is ordered before and after it.
 void ensure_stream(dfjk_handle* h) {
-    if (!h->own_stream) hcheck(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (!h->own_stream) hcheck(hipStreamCreateWithFlags(&h->own_stream, hipStreamDefault), "hipStreamCreate");
     if (!h->stream) h->stream = h->own_stream;
     h->timer.s = h->stream;
 }
@@ -402,11 +404,17 @@ void invalidate_copies(dfjk_handle* h) {
 }
 
 // One density pair: Cl/Cr are device [nbf][ncol] row-major (ld = ncol).
-// Cr == nullptr: lr_symmetric.  If every column of Cr is bitwise +/- the same column
-// of Cl (the incremental C_left=[C_new,C_old], C_right=[C_new,-C_old] form), K is
-// built as sum_Q X_Q diag(s) X_Q^T: the columns are regrouped (+ first, then -), one
-// half-transform is run and the upper block-triangle is contracted twice, with +1
-// over the + group and -1 over the - group.
+// Cr == nullptr: lr_symmetric.  Two structured non-symmetric forms are recognised
+// (bitwise column tests on the device) and kept symmetric:
+//  * signed: every column of Cr is +/- the same column of Cl (the incremental
+//    C_left=[C_new,C_old], C_right=[C_new,-C_old] form).  K = sum_Q X_Q diag(s) X_Q^T:
+//    the columns are regrouped (+ first, then -), one half-transform is run and the
+//    upper block-triangle is contracted twice, with +1 over the + group and -1 over the -.
+//  * cross: Cl = [A, V], Cr = [V, A] (ncol even; dD = A V^T + V A^T, the form
+//    A = (C_new + C_old)/2, V = C_new - C_old of a density change, in which every
+//    product carries the small factor V).  K = sum_Q X_A X_V^T + X_V X_A^T: one
+//    half-transform of [A, V], two contractions of the upper block-triangle with the
+//    column halves swapped between the operands.
 void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol, double* dD, double* dJ, double* dK,
                   int accumulate) {
     const TileGeom g = h->geom();
@@ -418,30 +426,43 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     EventTimer& tm = h->timer;
 
     // classify the pair and build the column map
-    bool sym = (dCr == nullptr);
+    bool sym = (dCr == nullptr), cross = false;
     std::vector<int> hs(ncol, 1);
     if (!sym) {
         int8_t* sg = h->ws_sign.get<int8_t>(ncol);
-        launch_column_signs(dCl, dCr, ncol, nbf, ncol, sg, s);
         std::vector<int8_t> h8(ncol);
-        hcheck(hipMemcpyAsync(h8.data(), sg, ncol, hipMemcpyDeviceToHost, s), "signs");
-        hcheck(hipStreamSynchronize(s), "signs");
+        auto signs = [&](int shift) {
+            launch_column_signs(dCl, dCr, ncol, nbf, ncol, shift, sg, s);
+            hcheck(hipMemcpyAsync(h8.data(), sg, ncol, hipMemcpyDeviceToHost, s), "signs");
+            hcheck(hipStreamSynchronize(s), "signs");
+        };
+        signs(0);
         bool allpm = true;
         for (int i = 0; i < ncol; ++i) {
             hs[i] = h8[i];
             allpm &= h8[i] != 0;
         }
         sym = allpm;
+        if (!sym && ncol % 2 == 0) {
+            signs(ncol / 2);
+            cross = true;
+            for (int i = 0; i < ncol; ++i) cross &= h8[i] == 1;
+            if (cross) std::fill(hs.begin(), hs.end(), 1);
+        }
     }
-    const bool general = !sym;
+    const bool general = !sym && !cross;
     std::vector<int> plus, minus;
     for (int i = 0; i < ncol; ++i) (hs[i] >= 0 ? plus : minus).push_back(i);
     const bool signed_sym = sym && !minus.empty();
-    int ncol_eff = ncol, kplus = 0;
+    int ncol_eff = ncol, kplus = 0, half = 0, wcross = 0;
     std::vector<int> cmap;
     if (signed_sym) {
         kplus = round_up(static_cast<int>(plus.size()), 32);
         ncol_eff = kplus + round_up(static_cast<int>(minus.size()), 32);
+    } else if (cross) {
+        half = ncol / 2;
+        wcross = round_up(half, 32);
+        ncol_eff = 2 * wcross;
     }
     int nbw, npanel, ldc;
     ht_panel_shape(ncol_eff, &nbw, &npanel, &ldc);
@@ -449,6 +470,11 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     if (signed_sym) {
         for (size_t i = 0; i < plus.size(); ++i) cmap[i] = plus[i];
         for (size_t i = 0; i < minus.size(); ++i) cmap[kplus + i] = minus[i];
+    } else if (cross) {
+        for (int i = 0; i < half; ++i) {
+            cmap[i] = i;
+            cmap[wcross + i] = half + i;
+        }
     } else {
         for (int i = 0; i < ncol; ++i) cmap[i] = i;
     }
@@ -476,8 +502,22 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         tm.end(T_OTHER, e0);
     }
 
-    // J (dfhelper.cc:3162-3285): d_Q = <B_Q, D>, J = sum_Q B_Q d_Q, FP64 at every precision
+    // J (dfhelper.cc:3162-3285): d_Q = <B_Q, D>, J = sum_Q B_Q d_Q, FP64 accumulation.  The
+    // tensor is read from the copy that matches the precision setting: FP64, the FP32 copy,
+    // or hi + lo of the BF16 pair (half the bytes of the FP64 pass for the mixed builds).
     if (h->do_J && dJ) {
+        JSource js;
+        if (h->precision == DFJK_PREC_FP32) {
+            js.kind = JSource::F32;
+            js.a = ensure_B32(h);
+        } else if (h->precision == DFJK_PREC_BF16X3) {
+            ensure_Bbf(h);
+            js.kind = JSource::BF2;
+            js.a = h->dBh.p;
+            js.b = h->dBl.p;
+        } else {
+            js.a = h->dB.p;
+        }
         hipEvent_t e0 = tm.begin();
         double* W = h->ws_W.get<double>(static_cast<size_t>(h->ntp) * TILE2);
         double* dpart = h->ws_dpart.get<double>(static_cast<size_t>(h->ntp) * h->naux);
@@ -485,14 +525,14 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         launch_pack_w(g, Dm, nbf, W, s);
         if (q0 > 0 || q1 < h->naux)
             hcheck(hipMemsetAsync(dpart, 0, sizeof(double) * static_cast<size_t>(h->ntp) * h->naux, s), "memset");
-        launch_j1(g, h->dB.as<double>(), W, h->naux, q0, q1, dpart, s);
+        launch_j1(g, js, W, h->naux, q0, q1, dpart, s);
         launch_j1_reduce(g, dpart, h->naux, dvec, s);
         const int want_wg = 4096;
         int nchunk = std::max(1, std::min(nq_total, (want_wg + h->ntp - 1) / h->ntp));
         const int qchunk = (h->naux + nchunk - 1) / nchunk;
         nchunk = (h->naux + qchunk - 1) / qchunk;
         double* Jpart = h->ws_Jpart.get<double>(static_cast<size_t>(nchunk) * h->ntp * TILE2);
-        launch_j2(g, h->dB.as<double>(), dvec, h->naux, qchunk, Jpart, s);
+        launch_j2(g, js, dvec, h->naux, qchunk, Jpart, s);
         launch_j_unpack(g, Jpart, nchunk, dJ, nbf, 1.0, accumulate, s);
         tm.end(T_J, e0);
     }
@@ -513,21 +553,26 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         qblk = std::min(qblk, 65535);
         const int nblocks = (nq_total + qblk - 1) / qblk;
         qblk = (nq_total + nblocks - 1) / nblocks;
-        const int ntile = sym ? h->ntile_sym : h->ntile_all;
-        const int2* otiles = sym ? h->d_otiles_sym.as<int2>() : h->d_otiles_all.as<int2>();
+        const bool upper = sym || cross;  // symmetric K: contract the upper block-triangle only
+        const int ntile = upper ? h->ntile_sym : h->ntile_all;
+        const int2* otiles = upper ? h->d_otiles_sym.as<int2>() : h->d_otiles_all.as<int2>();
         const int nsplit = choose_nsplit(ntile, qblk);
         double* P = h->ws_P.get<double>(static_cast<size_t>(nsplit) * ntile * 16384);
-        // contraction passes over k: (kbeg, kend, alpha)
+        // contraction passes over k: (kbeg, kend, alpha, column offsets of the two operands)
         struct Pass {
             int kbeg, kend;
             double alpha;
+            size_t offl, offr;
         };
         std::vector<Pass> passes;
         if (signed_sym) {
-            if (kplus > 0) passes.push_back({0, kplus, 1.0});
-            passes.push_back({kplus, ncol_eff, -1.0});
+            if (kplus > 0) passes.push_back({0, kplus, 1.0, 0, 0});
+            passes.push_back({kplus, ncol_eff, -1.0, 0, 0});
+        } else if (cross) {
+            passes.push_back({0, wcross, 1.0, 0, static_cast<size_t>(wcross)});
+            passes.push_back({0, wcross, 1.0, static_cast<size_t>(wcross), 0});
         } else {
-            passes.push_back({0, ldc, 1.0});
+            passes.push_back({0, ldc, 1.0, 0, 0});
         }
 
         const float *B32 = nullptr, *c32l = nullptr, *c32r = nullptr;
@@ -590,14 +635,18 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                 const Pass& pp = passes[ip];
                 const int acc = (blk > 0 || ip > 0) ? 1 : 0;
                 if (prec == DFJK_PREC_FP64) {
-                    launch_kgemm2_f64(reinterpret_cast<double*>(XL), reinterpret_cast<double*>(XR), xq, xrows, ldc,
-                                      pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha, acc, s);
+                    double* xl = reinterpret_cast<double*>(XL) + pp.offl;
+                    double* xr = reinterpret_cast<double*>(XR) + pp.offr;
+                    launch_kgemm2_f64(xl, xr, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha,
+                                      acc, s);
                 } else if (prec == DFJK_PREC_FP32) {
-                    launch_kgemm2_f32(reinterpret_cast<float*>(XL), reinterpret_cast<float*>(XR), xq, xrows, ldc,
-                                      pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha, acc, s);
+                    float* xl = reinterpret_cast<float*>(XL) + pp.offl;
+                    float* xr = reinterpret_cast<float*>(XR) + pp.offr;
+                    launch_kgemm2_f32(xl, xr, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha,
+                                      acc, s);
                 } else {
-                    uint16_t* xl = reinterpret_cast<uint16_t*>(XL);
-                    uint16_t* xr = reinterpret_cast<uint16_t*>(XR);
+                    uint16_t* xl = reinterpret_cast<uint16_t*>(XL) + pp.offl;
+                    uint16_t* xr = reinterpret_cast<uint16_t*>(XR) + pp.offr;
                     launch_kgemm_bf3(xl, xl + xq * qblk, xr, xr + xq * qblk, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp,
                                      otiles, ntile, P, pp.alpha, acc, s);
                 }
@@ -610,8 +659,8 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             tm.end(T_KGEMM, e1);
         }
         hipEvent_t e2 = tm.begin();
-        launch_tile_reduce(P, nsplit, ntile, sym ? h->d_otile_of_sym.as<int>() : h->d_otile_of_all.as<int>(), h->nB,
-                           sym ? 1 : 0, nbf, nbf, dK, nbf, 1.0, accumulate, s);
+        launch_tile_reduce(P, nsplit, ntile, upper ? h->d_otile_of_sym.as<int>() : h->d_otile_of_all.as<int>(), h->nB,
+                           upper ? 1 : 0, nbf, nbf, dK, nbf, 1.0, accumulate, s);
         tm.end(T_OTHER, e2);
     }
 }

@@ -41,8 +41,27 @@ __global__ void k_pack_w(const double* __restrict__ D, int ldD, int nbf, const i
 
 // One wave per Q: the wave holds the W tile (16 doubles per lane) and reduces
 // <B_Q tile, W tile> with a butterfly.  grid = (ntp, ceil((q1-q0)/qblk)), 256 threads.
-__global__ __launch_bounds__(256) void k_j1(const double* __restrict__ B, const double* __restrict__ W, int ntp,
-                                            int naux, int q0, int q1, int qblk, double* __restrict__ dpart) {
+// Element loaders of the resident tensor for the J path (FP64 accumulation in every case).
+struct BSrcF64 {
+    const double* p;
+    __device__ double operator()(size_t i) const { return p[i]; }
+};
+struct BSrcF32 {
+    const float* p;
+    __device__ double operator()(size_t i) const { return static_cast<double>(p[i]); }
+};
+struct BSrcBF2 {  // B ~ hi + lo, two BF16 copies
+    const uint16_t* h;
+    const uint16_t* l;
+    __device__ static double f(uint16_t v) { return static_cast<double>(__uint_as_float(static_cast<uint32_t>(v) << 16)); }
+    __device__ double operator()(size_t i) const { return f(h[i]) + f(l[i]); }
+};
+
+// d_Q partials: one wave per Q, the density tile of pair t held in registers.
+// grid = (ntp, ceil((q1-q0)/qblk)), 256 threads.
+template <class BS>
+__global__ __launch_bounds__(256) void k_j1(BS B, const double* __restrict__ W, int ntp, int naux, int q0, int q1,
+                                            int qblk, double* __restrict__ dpart) {
     const int t = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double* Wt = W + static_cast<size_t>(t) * TILE2;
@@ -52,10 +71,10 @@ __global__ __launch_bounds__(256) void k_j1(const double* __restrict__ B, const 
     const int qa = q0 + blockIdx.y * qblk;
     const int qb = min(q1, qa + qblk);
     for (int Q = qa + wave; Q < qb; Q += 4) {
-        const double* Bt = B + (static_cast<size_t>(Q) * ntp + t) * TILE2;
+        const size_t base = (static_cast<size_t>(Q) * ntp + t) * TILE2;
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) s = fma(Bt[lane + 64 * k], w[k], s);
+        for (int k = 0; k < 16; ++k) s = fma(B(base + lane + 64 * k), w[k], s);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
         if (lane == 0) dpart[static_cast<size_t>(t) * naux + Q] = s;
@@ -70,9 +89,10 @@ __global__ void k_j1_reduce(const double* __restrict__ dpart, int ntp, int naux,
     d[Q] = s;
 }
 
-// grid = (ntp, nchunk), 256 threads, 4 elements per thread.
-__global__ __launch_bounds__(256) void k_j2(const double* __restrict__ B, const double* __restrict__ d, int ntp,
-                                            int naux, int qchunk, double* __restrict__ Jpart) {
+// J partials over a chunk of Q: grid = (ntp, nchunk), 256 threads, 4 elements per thread.
+template <class BS>
+__global__ __launch_bounds__(256) void k_j2(BS B, const double* __restrict__ d, int ntp, int naux, int qchunk,
+                                            double* __restrict__ Jpart) {
     const int t = blockIdx.x, c = blockIdx.y;
     const int qa = c * qchunk, qb = min(naux, qa + qchunk);
     const int e0 = threadIdx.x;
@@ -80,11 +100,11 @@ __global__ __launch_bounds__(256) void k_j2(const double* __restrict__ B, const 
 #pragma unroll 2
     for (int Q = qa; Q < qb; ++Q) {
         const double dq = d[Q];
-        const double* Bt = B + (static_cast<size_t>(Q) * ntp + t) * TILE2 + e0;
-        a0 = fma(Bt[0], dq, a0);
-        a1 = fma(Bt[256], dq, a1);
-        a2 = fma(Bt[512], dq, a2);
-        a3 = fma(Bt[768], dq, a3);
+        const size_t base = (static_cast<size_t>(Q) * ntp + t) * TILE2 + e0;
+        a0 = fma(B(base), dq, a0);
+        a1 = fma(B(base + 256), dq, a1);
+        a2 = fma(B(base + 512), dq, a2);
+        a3 = fma(B(base + 768), dq, a3);
     }
     double* o = Jpart + (static_cast<size_t>(c) * ntp + t) * TILE2 + e0;
     o[0] = a0;
@@ -320,23 +340,39 @@ void launch_pack_w(const TileGeom& g, const double* D, int ldD, double* W, hipSt
     hipLaunchKernelGGL(k_pack_w, dim3(g.ntp), dim3(256), 0, s, D, ldD, g.nbf, g.d_tiles, W);
 }
 
-void launch_j1(const TileGeom& g, const double* B, const double* W, int naux, int q0, int q1, double* dpart,
+template <class F>
+void with_bsrc(const JSource& b, F&& f) {
+    if (b.kind == JSource::F64)
+        f(BSrcF64{static_cast<const double*>(b.a)});
+    else if (b.kind == JSource::F32)
+        f(BSrcF32{static_cast<const float*>(b.a)});
+    else
+        f(BSrcBF2{static_cast<const uint16_t*>(b.a), static_cast<const uint16_t*>(b.b)});
+}
+
+void launch_j1(const TileGeom& g, const JSource& B, const double* W, int naux, int q0, int q1, double* dpart,
                hipStream_t s) {
     require(q0 >= 0 && q1 <= naux && q0 < q1, "j1 q range");
+    require(B.a && (B.kind != JSource::BF2 || B.b), "j1 source");
     const int qblk = 64;
     dim3 grid(g.ntp, (q1 - q0 + qblk - 1) / qblk);
-    hipLaunchKernelGGL(k_j1, grid, dim3(256), 0, s, B, W, g.ntp, naux, q0, q1, qblk, dpart);
+    with_bsrc(B, [&](auto src) {
+        hipLaunchKernelGGL(k_j1<decltype(src)>, grid, dim3(256), 0, s, src, W, g.ntp, naux, q0, q1, qblk, dpart);
+    });
 }
 
 void launch_j1_reduce(const TileGeom& g, const double* dpart, int naux, double* d, hipStream_t s) {
     hipLaunchKernelGGL(k_j1_reduce, grid1(naux, 256), dim3(256), 0, s, dpart, g.ntp, naux, d);
 }
 
-void launch_j2(const TileGeom& g, const double* B, const double* d, int naux, int qchunk, double* Jpart,
+void launch_j2(const TileGeom& g, const JSource& B, const double* d, int naux, int qchunk, double* Jpart,
                hipStream_t s) {
     require(qchunk > 0, "j2 qchunk");
+    require(B.a && (B.kind != JSource::BF2 || B.b), "j2 source");
     dim3 grid(g.ntp, (naux + qchunk - 1) / qchunk);
-    hipLaunchKernelGGL(k_j2, grid, dim3(256), 0, s, B, d, g.ntp, naux, qchunk, Jpart);
+    with_bsrc(B, [&](auto src) {
+        hipLaunchKernelGGL(k_j2<decltype(src)>, grid, dim3(256), 0, s, src, d, g.ntp, naux, qchunk, Jpart);
+    });
 }
 
 void launch_j_unpack(const TileGeom& g, const double* Jpart, int nchunk, double* J, int ldJ, double alpha,

@@ -25,11 +25,17 @@ struct TileGeom {
 // W = tile-packed symmetrised density: I<J tiles hold D_mn + D_nm, diagonal tiles D_mn.
 void launch_pack_w(const TileGeom& g, const double* D, int ldD, double* W, hipStream_t s);
 // dpart[t*naux+Q] = <B_Q tile t, W tile t> for Q in [q0, q1)
-void launch_j1(const TileGeom& g, const double* B, const double* W, int naux, int q0, int q1, double* dpart,
+// Which copy of the resident tensor the J kernels read; every variant accumulates in FP64.
+struct JSource {
+    enum Kind { F64 = 0, F32 = 1, BF2 = 2 } kind = F64;
+    const void* a = nullptr;  // FP64 / FP32 tensor, or the BF16 hi copy
+    const void* b = nullptr;  // BF16 lo copy (BF2 only)
+};
+void launch_j1(const TileGeom& g, const JSource& B, const double* W, int naux, int q0, int q1, double* dpart,
                hipStream_t s);
 void launch_j1_reduce(const TileGeom& g, const double* dpart, int naux, double* d, hipStream_t s);
 // Jpart[c][t][e] = sum_{Q in chunk c} B[Q][t][e] d[Q]   (chunks of qchunk over [0,naux))
-void launch_j2(const TileGeom& g, const double* B, const double* d, int naux, int qchunk, double* Jpart,
+void launch_j2(const TileGeom& g, const JSource& B, const double* d, int naux, int qchunk, double* Jpart,
                hipStream_t s);
 // J[m][n] = alpha * sum_c Jpart[c][tile(m,n)] (+ J if accumulate)
 void launch_j_unpack(const TileGeom& g, const double* Jpart, int nchunk, double* J, int ldJ, double alpha,

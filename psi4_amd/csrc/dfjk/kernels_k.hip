@@ -365,7 +365,9 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const uint16_t* Ch = Al + A_SZ;
         const uint16_t* Cl = Ch + C_SZ;
         const int k8 = 8 * (lane >> 4);
-        bf8_t ah[4], al[4];
+        // all fragments of the step first (one LDS wait), then the MFMAs product by
+        // product, so consecutive updates of one accumulator are 4*NBW MFMAs apart
+        bf8_t ah[4], al[4], bh[NBW], bl[NBW];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int row = wm * 64 + i * 16 + (lane & 15);
@@ -375,15 +377,21 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int j = 0; j < NBW; ++j) {
             const int col = wn * NBW * 16 + j * 16 + (lane & 15);
-            const bf8_t bh = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ch + col * KP + k8));
-            const bf8_t bl = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Cl + col * KP + k8));
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
-            }
+            bh[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ch + col * KP + k8));
+            bl[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Cl + col * KP + k8));
         }
+#pragma unroll
+        for (int j = 0; j < NBW; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < NBW; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < NBW; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
     };
 
     if (nsteps > 0) {
@@ -524,14 +532,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 // =====================================================================================
 // K contraction, 3-term BF16 split on hi/lo X: A S B^T ~ Ah Bh + Ah Bl + Al Bh.
-// 128x128 tiles, 4 waves of 64x64, K-step 32, double-buffered LDS.
+// 128x128 tiles, 4 waves of 64x64, K-step 32, double-buffered LDS.  LDS rows are
+// 64 bytes (32 k) with the four 16-byte chunks of row r stored at chunk c ^ ((r>>1)&3):
+// the fragment reads (16 rows x one chunk per 16 lanes) and the staging writes are
+// bank-conflict free, and the 64 KB footprint leaves room for two workgroups per CU.
 // =====================================================================================
-__global__ __launch_bounds__(256) void k_kgemm_bf3(const uint16_t* __restrict__ Ah_, const uint16_t* __restrict__ Al_,
-                                                   const uint16_t* __restrict__ Bh_, const uint16_t* __restrict__ Bl_,
-                                                   size_t stride, int ld, int kbeg, int kend, int nbatch,
-                                                   int nsplit, const int2* __restrict__ tiles, int ntile,
-                                                   double* __restrict__ P, double alpha, int accumulate) {
-    constexpr int KS = 32, KP = 40, SZ = 128 * KP;
+__device__ inline int kg_swz(int row, int chunk) { return row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3); }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_bf3(
+    const uint16_t* __restrict__ Ah_, const uint16_t* __restrict__ Al_, const uint16_t* __restrict__ Bh_,
+    const uint16_t* __restrict__ Bl_, size_t stride, int ld, int kbeg, int kend, int nbatch, int nsplit,
+    const int2* __restrict__ tiles, int ntile, double* __restrict__ P, double alpha, int accumulate) {
+    constexpr int KS = 32, SZ = 128 * 32;
     __shared__ uint16_t Ls[2][4 * SZ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
@@ -550,8 +562,8 @@ __global__ __launch_bounds__(256) void k_kgemm_bf3(const uint16_t* __restrict__ 
     for (int i = 0; i < 16; ++i) acc[i] = f4_t{0, 0, 0, 0};
 
     // per step: 4 arrays x 128 rows x 32 k = 4 x 512 vectors of 8; 256 threads -> 2 vectors per array
-    // thread's two 8-element vectors per array: rows (tid>>2) and (tid>>2)+64, k8 = 8*(tid&3)
-    const int vrow = tid >> 2, vk8 = 8 * (tid & 3);
+    // thread's two 8-element vectors per array: rows (tid>>2) and (tid>>2)+64, chunk tid&3
+    const int vrow = tid >> 2, vc = tid & 3, vk8 = 8 * vc;
     uint4 a0h, a0l, b0h, b0l, a1h, a1l, b1h, b1l;
     auto gload = [&](int s) {
         const int b = b0 + s / nks;
@@ -570,9 +582,9 @@ __global__ __launch_bounds__(256) void k_kgemm_bf3(const uint16_t* __restrict__ 
         b1h = *reinterpret_cast<const uint4*>(Bh_ + ob1);
         b1l = *reinterpret_cast<const uint4*>(Bl_ + ob1);
     };
+    const int o0 = kg_swz(vrow, vc), o1 = kg_swz(vrow + 64, vc);
     auto lstore = [&](int buf) {
         uint16_t* L = Ls[buf];
-        const int o0 = vrow * KP + vk8, o1 = (vrow + 64) * KP + vk8;
         *reinterpret_cast<uint4*>(L + o0) = a0h;
         *reinterpret_cast<uint4*>(L + SZ + o0) = a0l;
         *reinterpret_cast<uint4*>(L + 2 * SZ + o0) = b0h;
@@ -582,28 +594,34 @@ __global__ __launch_bounds__(256) void k_kgemm_bf3(const uint16_t* __restrict__ 
         *reinterpret_cast<uint4*>(L + 2 * SZ + o1) = b1h;
         *reinterpret_cast<uint4*>(L + 3 * SZ + o1) = b1l;
     };
+    const int fc = lane >> 4;
     auto compute = [&](int buf) {
         const uint16_t* L = Ls[buf];
-        const int k8 = 8 * (lane >> 4);
-        bf8_t ah[4], al[4];
+        bf8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int row = wr * 64 + i * 16 + (lane & 15);
-            ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + row * KP + k8));
-            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + SZ + row * KP + k8));
+            const int ra = kg_swz(wr * 64 + i * 16 + (lane & 15), fc);
+            const int rb = kg_swz(wc * 64 + i * 16 + (lane & 15), fc);
+            ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + ra));
+            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + SZ + ra));
+            bh[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + 2 * SZ + rb));
+            bl[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + 3 * SZ + rb));
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int col = wc * 64 + j * 16 + (lane & 15);
-            const bf8_t bh = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + 2 * SZ + col * KP + k8));
-            const bf8_t bl = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + 3 * SZ + col * KP + k8));
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i * 4 + j], 0, 0, 0);
-                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i * 4 + j], 0, 0, 0);
-                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i * 4 + j], 0, 0, 0);
-            }
-        }
+            for (int i = 0; i < 4; ++i)
+                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i * 4 + j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i * 4 + j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                acc[i * 4 + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i * 4 + j], 0, 0, 0);
     };
 
     if (nsteps > 0) {
@@ -678,9 +696,11 @@ __global__ void k_pack_cT_bf(const double* __restrict__ src, int ldsrc, int nbf,
 
 // Column signs: s[i] = +1 if Cr[:,i] == Cl[:,i], -1 if Cr[:,i] == -Cl[:,i], 0 otherwise
 // (only bitwise-exact matches count).  One block per column.
+// s[i] = +1 / -1 when column i of Cr is bitwise +/- column (i + shift) % ncol of Cl, else 0.
 __global__ void k_column_signs(const double* __restrict__ Cl, const double* __restrict__ Cr, int ld, int nbf,
-                               int8_t* __restrict__ s) {
+                               int ncol, int shift, int8_t* __restrict__ s) {
     const int i = blockIdx.x;
+    const int il = (i + shift) % ncol;
     __shared__ int plus, minus;
     if (threadIdx.x == 0) {
         plus = 1;
@@ -689,7 +709,7 @@ __global__ void k_column_signs(const double* __restrict__ Cl, const double* __re
     __syncthreads();
     int p = 1, mi = 1;
     for (int m = threadIdx.x; m < nbf; m += blockDim.x) {
-        const double a = Cl[static_cast<size_t>(m) * ld + i], b = Cr[static_cast<size_t>(m) * ld + i];
+        const double a = Cl[static_cast<size_t>(m) * ld + il], b = Cr[static_cast<size_t>(m) * ld + i];
         p &= (a == b);
         mi &= (a == -b);
     }
@@ -843,9 +863,10 @@ void launch_pack_cT_bf(const double* src, int ldsrc, int nbf, const int* cmap, u
     hipLaunchKernelGGL(k_pack_cT_bf, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, src, ldsrc, nbf,
                        cmap, hi, lo, nbfp, ldc);
 }
-void launch_column_signs(const double* Cl, const double* Cr, int ld, int nbf, int ncol, int8_t* s, hipStream_t st) {
-    require(ncol > 0 && ld >= ncol, "column_signs");
-    hipLaunchKernelGGL(k_column_signs, dim3(ncol), dim3(256), 0, st, Cl, Cr, ld, nbf, s);
+void launch_column_signs(const double* Cl, const double* Cr, int ld, int nbf, int ncol, int shift, int8_t* s,
+                         hipStream_t st) {
+    require(ncol > 0 && ld >= ncol && shift >= 0 && shift < ncol, "column_signs");
+    hipLaunchKernelGGL(k_column_signs, dim3(ncol), dim3(256), 0, st, Cl, Cr, ld, nbf, ncol, shift, s);
 }
 
 }  // namespace p4a

@@ -47,7 +47,8 @@ void launch_pack_cmap_f32(const double* src, int ldsrc, int nbf, const int* cmap
                           hipStream_t s);
 void launch_pack_cT_bf(const double* src, int ldsrc, int nbf, const int* cmap, uint16_t* hi, uint16_t* lo, int nbfp,
                        int ldc, hipStream_t s);
-// s[i] = +1 / -1 if Cr[:,i] is bitwise +/- Cl[:,i], else 0.
-void launch_column_signs(const double* Cl, const double* Cr, int ld, int nbf, int ncol, int8_t* s, hipStream_t st);
+// s[i] = +1 / -1 if Cr[:,i] is bitwise +/- Cl[:,(i + shift) % ncol], else 0.
+void launch_column_signs(const double* Cl, const double* Cr, int ld, int nbf, int ncol, int shift, int8_t* s,
+                         hipStream_t st);
 
 }  // namespace p4a

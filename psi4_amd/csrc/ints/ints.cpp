@@ -478,7 +478,8 @@ void boys(int nmax, double T, double* F) {
 namespace {
 
 // Transform a [nca][ncb] Cartesian one-electron block to functions and scatter.
-void scatter_1e(const Basis& b, int I, int J, const std::vector<double>& blk, double* M) {
+// transpose_sign = +1 for Hermitian operators, -1 for antisymmetric ones (r x nabla).
+void scatter_1e(const Basis& b, int I, int J, const std::vector<double>& blk, double* M, double transpose_sign = 1.0) {
     const Shell& si = b.shells[I];
     const Shell& sj = b.shells[J];
     Work w;
@@ -490,8 +491,10 @@ void scatter_1e(const Basis& b, int I, int J, const std::vector<double>& blk, do
     for (int a = 0; a < ni; ++a)
         for (int c = 0; c < nj; ++c) {
             double v = w.cart[static_cast<size_t>(a) * nj + c];
+            // within a diagonal block the later (c,a) visit overwrites both elements, so the
+            // matrix is exactly (anti)symmetric
             M[static_cast<size_t>(b.offset[I] + a) * b.nbf + b.offset[J] + c] = v;
-            M[static_cast<size_t>(b.offset[J] + c) * b.nbf + b.offset[I] + a] = v;
+            M[static_cast<size_t>(b.offset[J] + c) * b.nbf + b.offset[I] + a] = transpose_sign * v;
         }
 }
 
@@ -612,8 +615,61 @@ void dipole_ints(const Basis& b, const double* O, double* M) {
     }
 }
 
+// Angular momentum integrals <a| (r - O) x nabla |b> (no factor -i), d = x, y, z, into
+// M[3][nbf][nbf] (antisymmetric).  Per Cartesian direction, with the 1D factors of the
+// Hermite expansion (common (pi/p)^3/2 prefactor taken out):
+//   S(i,j) = E^{ij}_0,   M(i,j) = E^{ij}_1 + (P - O) E^{ij}_0          (moment x - O_x)
+//   D(i,j) = j S(i,j-1) - 2 beta S(i,j+1)                              (d/dx on the ket)
+//   L_x = S_x (M_y D_z - D_y M_z), and cyclic.
+void angmom_ints(const Basis& b, const double* O, double* M) {
+    const int ns = static_cast<int>(b.shells.size());
+    const size_t nn = static_cast<size_t>(b.nbf) * b.nbf;
+#pragma omp parallel for schedule(dynamic)
+    for (int I = 0; I < ns; ++I) {
+        for (int J = 0; J <= I; ++J) {
+            const Shell& A = b.shells[I];
+            const Shell& B = b.shells[J];
+            Shell Bx = B;
+            Bx.l = B.l + 1;  // E tables up to lb+1 for the derivative
+            PairData pd = make_pair(A, Bx);
+            const auto& ca = carts().comps[A.l];
+            const auto& cb = carts().comps[B.l];
+            std::vector<double> blk[3];
+            for (auto& v : blk) v.assign(ca.size() * cb.size(), 0.0);
+            const int st = pd.lab + 1;
+            for (size_t k = 0; k < pd.prims.size(); ++k) {
+                const PrimPair& pp = pd.prims[k];
+                const double* E = pd.Eprim(k);
+                const double s0 = std::pow(kPi / pp.p, 1.5) * pp.pref;
+                const double beta = pp.eb;
+                auto e = [&](int d, int i, int j, int t) -> double {
+                    if (j < 0 || t > i + j) return 0.0;
+                    return E[d * pd.esz + (i * (pd.lb + 1) + j) * st + t];
+                };
+                size_t kk = 0;
+                for (auto& a : ca)
+                    for (auto& bb : cb) {
+                        double S[3], Mo[3], Dv[3];
+                        for (int d = 0; d < 3; ++d) {
+                            const int i = a[d], j = bb[d];
+                            S[d] = e(d, i, j, 0);
+                            Mo[d] = e(d, i, j, 1) + (pp.P[d] - O[d]) * e(d, i, j, 0);
+                            Dv[d] = j * e(d, i, j - 1, 0) - 2.0 * beta * e(d, i, j + 1, 0);
+                        }
+                        blk[0][kk] += s0 * S[0] * (Mo[1] * Dv[2] - Dv[1] * Mo[2]);
+                        blk[1][kk] += s0 * S[1] * (Mo[2] * Dv[0] - Dv[2] * Mo[0]);
+                        blk[2][kk] += s0 * S[2] * (Mo[0] * Dv[1] - Dv[0] * Mo[1]);
+                        ++kk;
+                    }
+            }
+            for (int d = 0; d < 3; ++d) scatter_1e(b, I, J, blk[d], M + d * nn, -1.0);
+        }
+    }
+}
+
 }  // namespace
 
+void angular_momentum(const Basis& b, const double* origin, double* M) { angmom_ints(b, origin, M); }
 void overlap(const Basis& b, double* S) { one_electron(b, OneE::S, 0, nullptr, nullptr, S); }
 void dipole(const Basis& b, const double* origin, double* M) { dipole_ints(b, origin, M); }
 void kinetic(const Basis& b, double* T) {

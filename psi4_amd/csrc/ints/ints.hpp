@@ -55,6 +55,8 @@ void overlap(const Basis& b, double* S);
 void kinetic(const Basis& b, double* T);
 // <mu| r_d - origin_d |nu> for d = x, y, z into M[3][nbf][nbf] (no electron charge).
 void dipole(const Basis& b, const double* origin, double* M);
+// <mu| (r - origin) x nabla |nu> for x, y, z into M[3][nbf][nbf] (antisymmetric; no factor -i).
+void angular_momentum(const Basis& b, const double* origin, double* M);
 void potential(const Basis& b, int natom, const double* Z, const double* xyz, double* V);
 // (P|Q), naux x naux.
 void metric(const Basis& aux, double* J, int nthreads);

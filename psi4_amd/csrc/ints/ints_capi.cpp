@@ -48,6 +48,13 @@ int p4a_ints_dipole(const p4a_basis* b, const double* origin, double* M) {
     });
 }
 
+int p4a_ints_angular_momentum(const p4a_basis* b, const double* origin, double* M) {
+    return guarded([&] {
+        const double zero[3] = {0.0, 0.0, 0.0};
+        p4a::angular_momentum(p4a::basis_from_desc(b), origin ? origin : zero, M);
+    });
+}
+
 int p4a_ints_potential(const p4a_basis* b, int natom, const double* Z, const double* xyz, double* V) {
     return guarded([&] { p4a::potential(p4a::basis_from_desc(b), natom, Z, xyz, V); });
 }

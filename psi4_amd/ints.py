@@ -53,6 +53,17 @@ def dipole(basis, origin=(0.0, 0.0, 0.0)):
     return M
 
 
+def angular_momentum(basis, origin=(0.0, 0.0, 0.0)):
+    """<mu| (r - origin) x nabla |nu> as [3, nbf, nbf] (antisymmetric; no factor -i)."""
+    lib = load_ints_lib()
+    n = basis.nbf()
+    M = np.zeros((3, n, n))
+    cb, _keep = basis.as_c()
+    o = np.ascontiguousarray(origin, np.float64)
+    _check(lib, lib.p4a_ints_angular_momentum(ctypes.byref(cb), _ptr(o), _ptr(M)))
+    return M
+
+
 def potential(basis, mol):
     lib = load_ints_lib()
     n = basis.nbf()

@@ -140,15 +140,19 @@ class RHF:
         return jk.J()[0].copy(), jk.K()[0].copy()
 
     def _delta_build(self, Cnew, Cold, S):
-        """J(dD), K(dD) for dD = Cnew Cnew^T - Cold Cold^T in the stable form."""
+        """J(dD), K(dD) for dD = Cnew Cnew^T - Cold Cold^T in the cross form
+        dD = A V^T + V A^T, A = (Cnew + Cold)/2, V = Cnew - Cold (C_left = [A, V],
+        C_right = [V, A]), after aligning Cold to Cnew: every low-precision product
+        carries the small factor V, and the engine keeps K(dD) symmetric."""
         u, _, vt = np.linalg.svd(Cold.T @ S @ Cnew)
         Col = Cold @ (u @ vt)            # same density, aligned with Cnew
-        dC = Cnew - Col
+        V = Cnew - Col
+        A = 0.5 * (Cnew + Col)
         jk = self.jk()
         jk.set_precision(_PREC[self.mixed])
         jk.C_clear()
-        jk.C_left_add(np.ascontiguousarray(np.hstack([Cnew, dC])))
-        jk.C_right_add(np.ascontiguousarray(np.hstack([dC, Col])))
+        jk.C_left_add(np.ascontiguousarray(np.hstack([A, V])))
+        jk.C_right_add(np.ascontiguousarray(np.hstack([V, A])))
         jk.compute()
         jk.set_precision(PREC_FP64)
         return jk.J()[0].copy(), jk.K()[0].copy()

@@ -1,9 +1,10 @@
 """GPU DF-JK through the C ABI vs the CPU oracle (restated MemDFJK).
 
-FP64 J and K must match the oracle to 1e-11 max-abs (the north-star bar for
-plain FP64).  Mixed-precision K is checked against the FP64 oracle with
-tolerances relative to max|K|: FP32 MFMA 2e-6, 3-term BF16 split 2e-4; J is
-FP64 at every precision and keeps the 1e-11 bar.
+FP64 J and K must match the oracle to 1e-11 max-abs (the bar for plain FP64).
+Mixed-precision K is checked against the FP64 oracle with tolerances relative
+to max|K|: FP32 MFMA 2e-6, 3-term BF16 split 2e-4.  In the mixed modes J
+accumulates in FP64 from the FP32 copy of B (1e-7 of max|J|) or from the BF16
+hi + lo pair (2e-5 of max|J|).
 """
 import numpy as np
 import pytest
@@ -15,6 +16,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-11
 REL_K = {1: 2e-6, 2: 2e-4}  # PREC_FP32, PREC_BF16X3
+# J accumulates in FP64 but reads the FP32 copy / the BF16 hi+lo pair of B in the mixed modes
+REL_J = {1: 1e-7, 2: 2e-5}
 
 
 @pytest.fixture(scope="module")
@@ -124,7 +127,7 @@ def test_mixed_precision_symmetric_h2o8(h2o8, prec):
     finally:
         jk.set_precision(0)
     Do, Jo, Ko = ora.compute([C])
-    assert maxabs(J[0], Jo[0]) < TOL
+    assert maxabs(J[0], Jo[0]) < REL_J[prec] * np.abs(Jo[0]).max()
     scale = np.abs(Ko[0]).max()
     assert maxabs(K[0], Ko[0]) < REL_K[prec] * scale
     assert maxabs(K[0], K[0].T) < 1e-14 * scale  # symmetric by construction
@@ -149,9 +152,40 @@ def test_mixed_precision_incremental_h2o8(h2o8, prec):
     _, Jn, Kn = ora.compute([Cn])
     _, Jold, Kold = ora.compute([Co])
     ref = Kn[0] - Kold[0]
-    assert maxabs(J[0], Jn[0] - Jold[0]) < TOL
+    dJ = Jn[0] - Jold[0]
+    assert maxabs(J[0], dJ) < 5 * REL_J[prec] * np.abs(dJ).max()
     # error relative to |dK| stays at the precision's relative level
     assert maxabs(K[0], ref) < 5 * REL_K[prec] * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_cross_incremental_h2o8(h2o8, prec):
+    """Cross form dD = A V^T + V A^T (C_left=[A, V], C_right=[V, A], A = (C_new + C_old)/2,
+    V = C_new - C_old): FP64 to 1e-11 against K(C_new) - K(C_old); mixed precision relative
+    to |dK|; K(dD) exactly symmetric at every precision."""
+    mol, prim, aux, ora, jk = h2o8
+    nbf = prim.nbf()
+    Cn = random_orbitals(nbf, 40, seed=41)
+    V = 1e-4 * random_orbitals(nbf, 40, seed=42)
+    Co = Cn - V
+    A = 0.5 * (Cn + Co)
+    announce(f"cross incremental form, precision {prec}, (H2O)_8")
+    jk.set_precision(prec)
+    try:
+        D, J, K = _run(jk, [np.hstack([A, V])], [np.hstack([V, A])])
+    finally:
+        jk.set_precision(0)
+    _, Jn, Kn = ora.compute([Cn])
+    _, Jold, Kold = ora.compute([Co])
+    ref = Kn[0] - Kold[0]
+    assert maxabs(D[0], Cn @ Cn.T - Co @ Co.T) < 1e-14
+    assert maxabs(K[0], K[0].T) == 0.0
+    if prec == 0:
+        assert maxabs(K[0], ref) < TOL and maxabs(J[0], Jn[0] - Jold[0]) < TOL
+    else:
+        assert maxabs(K[0], ref) < 5 * REL_K[prec] * np.abs(ref).max()
+        dJ = Jn[0] - Jold[0]
+        assert maxabs(J[0], dJ) < 5 * REL_J[prec] * np.abs(dJ).max()
 
 
 def test_accumulate_and_aux_range_h2o4(h2o4):
@@ -228,5 +262,5 @@ def test_mixed_two_panel_incremental_synthetic(gpu_available, prec, shape):
     print(f"  prec {prec} shape {shape}: |dK| max {scale_d:.3e}  err {err:.3e}  rel {err / scale_d:.3e}  "
           f"|K| max {np.abs(Kfull[0]).max():.3e}  |tr(D dK_err)| {dE:.3e}", flush=True)
     assert scale_d > 1e-6 * np.abs(Kfull[0]).max()
-    assert np.abs(out[prec][0] - out[0][0]).max() < 1e-11
+    assert np.abs(out[prec][0] - out[0][0]).max() < 5 * REL_J[prec] * np.abs(out[0][0]).max()
     assert err < 5 * REL_K[prec] * scale_d

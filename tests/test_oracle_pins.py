@@ -70,6 +70,22 @@ def test_psi4_jkmemory_ar5():
 
 
 # ------------------------------------------------------------------ closed forms
+def test_psi4_ao_angular_momentum_h2o():
+    """Element-wise pin of the AO conventions (normalisation, Cartesian and solid-harmonic
+    order, s/p/d) of the host integral engine: psi4's stored MintsHelper::ao_angular_momentum
+    of H2O / cc-pVDZ (tests/pytests/oei_reference_data.json, compared there at atol 1e-14).
+    psi4's matrices are -<mu|(r x nabla)|nu>."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "psi4_ao_angular_momentum_h2o_ccpvdz.json")))
+    mol = M.Molecule([a[0] for a in fx["geometry_bohr"]], np.array([a[1:] for a in fx["geometry_bohr"]]))
+    bs = BasisSet.build(mol, fx["basis"])
+    ref = np.array(fx["L"]).reshape(fx["L_shape"])
+    L = ints.angular_momentum(bs)
+    assert np.abs(-L - ref).max() < 1e-14
+    assert np.abs(ref).max() > 1.0
+
+
 @pytest.mark.parametrize("T", [0.0, 1e-8, 0.3, 2.5, 17.0, 44.9, 45.1, 120.0])
 def test_boys_F0_closed_form(T):
     F = ints.boys(12, T)
