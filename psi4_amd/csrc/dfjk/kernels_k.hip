@@ -165,8 +165,22 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int j = 0; j < NBW; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     T ra[NA], rc[NC];
+    // Column tile of the next step to load, read from the LDS step list one step
+    // ahead, so a step's loads wait on one LDS lookup (the block offsets) instead of two.
+    // The FP64 5-column-block shape has no register left for it and looks the tile up in place.
+    constexpr bool PF = !(F64 && NBW == 5);
+    int nJ = 0;
+    auto fetch = [&](int st) {
+        if constexpr (PF) nJ = sl.J[st / (TILE / KS)];
+    };
+    int lJ = 0;  // column tile of the step held in ra/rc
     auto gload = [&](int st) {
-        const int J = sl.J[st / (TILE / KS)];
+        int J;
+        if constexpr (PF)
+            J = nJ;
+        else
+            J = sl.J[st / (TILE / KS)];
+        lJ = J;
         const int kh = (st % (TILE / KS)) * KS;
 #pragma unroll
         for (int it = 0; it < NA; ++it) {
@@ -182,9 +196,14 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             const int kk = e / PAN, cc = e - kk * PAN;
             rc[it] = Cj[static_cast<size_t>(kk) * ldc + cc];
         }
+        if (st + 1 < nsteps) fetch(st + 1);
     };
     auto lstore = [&](int buf, int st) {
-        const int J = sl.J[st / (TILE / KS)];
+        int J;
+        if constexpr (PF)
+            J = lJ;
+        else
+            J = sl.J[st / (TILE / KS)];
 #pragma unroll
         for (int it = 0; it < NA; ++it) {
             const ACoord ac = a_coord<KS>(tid + NTHR * it, I0, J);
@@ -218,6 +237,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     };
 
     if (nsteps > 0) {
+        fetch(0);
         gload(0);
         lstore(0, 0);
         __syncthreads();
@@ -299,9 +319,15 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     uint4 rah, ral;
     uint4 c0h, c0l, c1h, c1l, c2h, c2l, c3h, c3l, c4h, c4l;
     int at_flag = 0;
+    // tile index of the next step to load, read one step ahead (see k_ht_mfma)
+    int nJ = 0, noff = -1;
+    auto fetch = [&](int st) {
+        nJ = sl.J[st];
+        noff = sl.off[nJ * RT + sub];
+    };
     auto gload = [&](int st) {
-        const int J = sl.J[st];
-        const int off = sl.off[J * RT + sub];
+        const int J = nJ;
+        const int off = noff;
         at_flag = (I0 + sub > J);
         const uint16_t* th = off >= 0 ? Bh + qoff + off : ztile;
         const uint16_t* tl = off >= 0 ? Bl + qoff + off : ztile;
@@ -324,6 +350,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         if (NCV >= 2) cload(2, c2h, c2l);
         if (NCV >= 3) cload(3, c3h, c3l);
         if (NCV >= 4) cload(4, c4h, c4l);
+        if (st + 1 < nsteps) fetch(st + 1);
     };
     auto lstore = [&](int buf) {
         uint16_t* Ah = Ls[buf];
@@ -395,6 +422,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     };
 
     if (nsteps > 0) {
+        fetch(0);
         gload(0);
         lstore(0);
         __syncthreads();
