@@ -52,6 +52,21 @@ int p4a_ints_metric(const p4a_basis* aux, double* J, int nthreads);
  * shell pairs with shell_mask[M*nshell+N]==0 are skipped (mask may be NULL). */
 int p4a_ints_three_index(const p4a_basis* prim, const p4a_basis* aux, const int64_t* dst, int64_t p_stride,
                          const uint8_t* shell_mask, double* out, int nthreads);
+/* Gradient contractions for the DF-SCF gradient (scfgrad/scf_grad.cc:103-262,
+ * jk_grad.cc:834-1000): derivatives with respect to the centre of each shell, added to
+ * g_shell [nshell*3] (and, for the nuclear attraction operator, to g_nuc [natom*3]).
+ *   overlap_grad:     sum_mn W_mn dS_mn/dR          (W symmetric, e.g. the energy-weighted density)
+ *   kinetic_grad:     sum_mn D_mn dT_mn/dR
+ *   potential_grad:   sum_mn D_mn dV_mn/dR, including the moving nuclei
+ *   three_index_grad: sum_{P,mn} G[P][m][n] d(P|mn)/dR  (G [naux][nbf][nbf], symmetric in m,n)
+ *   metric_grad:      sum_{PQ} W[P][Q] d(P|Q)/dR        (W symmetric) */
+int p4a_ints_overlap_grad(const p4a_basis* b, const double* W, double* g_shell);
+int p4a_ints_kinetic_grad(const p4a_basis* b, const double* D, double* g_shell);
+int p4a_ints_potential_grad(const p4a_basis* b, const double* D, int natom, const double* Z, const double* xyz,
+                            double* g_shell, double* g_nuc);
+int p4a_ints_three_index_grad(const p4a_basis* prim, const p4a_basis* aux, const double* G, double* g_prim,
+                              double* g_aux, int nthreads);
+int p4a_ints_metric_grad(const p4a_basis* aux, const double* W, double* g_aux, int nthreads);
 /* Schwarz diagonal (mn|mn): fun_max [nbf*nbf], shell_max [nshell*nshell], max_val
  * (dfhelper.cc:322-366). */
 int p4a_ints_schwarz(const p4a_basis* prim, double* fun_max, double* shell_max, double* max_val, int nthreads);

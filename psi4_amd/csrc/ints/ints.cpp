@@ -500,77 +500,186 @@ void scatter_1e(const Basis& b, int I, int J, const std::vector<double>& blk, do
 
 enum class OneE { S, T, V };
 
+// Cartesian block <A|O|B> ([ncart(A)][ncart(B)], coefficients as stored in the shells).
+// V sums over the nuclei [c0, c1).
+void onee_cart(const Shell& A, const Shell& B, OneE kind, int c0, int c1, const double* Z, const double* xyz,
+               std::vector<double>& blk, Work& w) {
+    Shell Bx = B;
+    if (kind == OneE::T) Bx.l = B.l + 2;  // E tables up to lb+2 for the Laplacian
+    PairData pd = make_pair(A, Bx);
+    const auto& ca = carts().comps[A.l];
+    const auto& cb = carts().comps[B.l];
+    blk.assign(ca.size() * cb.size(), 0.0);
+    const int st = pd.lab + 1;
+    for (size_t k = 0; k < pd.prims.size(); ++k) {
+        const PrimPair& pp = pd.prims[k];
+        const double* E = pd.Eprim(k);
+        double s0 = std::pow(kPi / pp.p, 1.5) * pp.pref;
+        if (kind == OneE::V) {
+            int L = pd.lab;
+            for (int c = c0; c < c1; ++c) {
+                double PC[3] = {pp.P[0] - xyz[3 * c], pp.P[1] - xyz[3 * c + 1], pp.P[2] - xyz[3 * c + 2]};
+                hermite_R(L, pp.p, PC, w);
+                double f = -Z[c] * 2.0 * kPi / pp.p * pp.pref;
+                size_t kk = 0;
+                for (auto& a : ca)
+                    for (auto& bb : cb) {
+                        const double* ex = E + (a[0] * (pd.lb + 1) + bb[0]) * st;
+                        const double* ey = E + pd.esz + (a[1] * (pd.lb + 1) + bb[1]) * st;
+                        const double* ez = E + 2 * pd.esz + (a[2] * (pd.lb + 1) + bb[2]) * st;
+                        double acc = 0.0;
+                        for (int t = 0; t <= a[0] + bb[0]; ++t)
+                            for (int u = 0; u <= a[1] + bb[1]; ++u)
+                                for (int v = 0; v <= a[2] + bb[2]; ++v)
+                                    acc += ex[t] * ey[u] * ez[v] * w.R[hidx(t, u, v)];
+                        blk[kk++] += f * acc;
+                    }
+            }
+        } else if (kind == OneE::S) {
+            size_t kk = 0;
+            for (auto& a : ca)
+                for (auto& bb : cb) {
+                    double sx = E[(a[0] * (pd.lb + 1) + bb[0]) * st];
+                    double sy = E[pd.esz + (a[1] * (pd.lb + 1) + bb[1]) * st];
+                    double sz = E[2 * pd.esz + (a[2] * (pd.lb + 1) + bb[2]) * st];
+                    blk[kk++] += s0 * sx * sy * sz;
+                }
+        } else {
+            const double beta = pp.eb;
+            size_t kk = 0;
+            auto s1 = [&](int d, int i, int j) -> double {
+                if (j < 0) return 0.0;
+                return E[d * pd.esz + (i * (pd.lb + 1) + j) * st];
+            };
+            for (auto& a : ca)
+                for (auto& bb : cb) {
+                    double s[3], t[3];
+                    for (int d = 0; d < 3; ++d) {
+                        int i = a[d], j = bb[d];
+                        s[d] = s1(d, i, j);
+                        t[d] = -0.5 * (4.0 * beta * beta * s1(d, i, j + 2) - 2.0 * beta * (2 * j + 1) * s1(d, i, j) +
+                                       j * (j - 1) * s1(d, i, j - 2));
+                    }
+                    blk[kk++] += s0 * (t[0] * s[1] * s[2] + s[0] * t[1] * s[2] + s[0] * s[1] * t[2]);
+                }
+        }
+    }
+}
+
 void one_electron(const Basis& b, OneE kind, int natom, const double* Z, const double* xyz, double* M) {
     const int ns = static_cast<int>(b.shells.size());
 #pragma omp parallel for schedule(dynamic)
     for (int I = 0; I < ns; ++I) {
         Work w;
+        std::vector<double> blk;
         for (int J = 0; J <= I; ++J) {
-            const Shell& A = b.shells[I];
-            const Shell& B = b.shells[J];
-            Shell Bx = B;
-            if (kind == OneE::T) Bx.l = B.l + 2;  // E tables up to lb+2 for the Laplacian
-            PairData pd = make_pair(A, Bx);
-            const auto& ca = carts().comps[A.l];
-            const auto& cb = carts().comps[B.l];
-            std::vector<double> blk(ca.size() * cb.size(), 0.0);
-            const int st = pd.lab + 1;
-            for (size_t k = 0; k < pd.prims.size(); ++k) {
-                const PrimPair& pp = pd.prims[k];
-                const double* E = pd.Eprim(k);
-                double s0 = std::pow(kPi / pp.p, 1.5) * pp.pref;
-                if (kind == OneE::V) {
-                    int L = pd.lab;
-                    for (int c = 0; c < natom; ++c) {
-                        double PC[3] = {pp.P[0] - xyz[3 * c], pp.P[1] - xyz[3 * c + 1], pp.P[2] - xyz[3 * c + 2]};
-                        hermite_R(L, pp.p, PC, w);
-                        double f = -Z[c] * 2.0 * kPi / pp.p * pp.pref;
-                        size_t kk = 0;
-                        for (auto& a : ca)
-                            for (auto& bb : cb) {
-                                const double* ex = E + (a[0] * (pd.lb + 1) + bb[0]) * st;
-                                const double* ey = E + pd.esz + (a[1] * (pd.lb + 1) + bb[1]) * st;
-                                const double* ez = E + 2 * pd.esz + (a[2] * (pd.lb + 1) + bb[2]) * st;
-                                double acc = 0.0;
-                                for (int t = 0; t <= a[0] + bb[0]; ++t)
-                                    for (int u = 0; u <= a[1] + bb[1]; ++u)
-                                        for (int v = 0; v <= a[2] + bb[2]; ++v)
-                                            acc += ex[t] * ey[u] * ez[v] * w.R[hidx(t, u, v)];
-                                blk[kk++] += f * acc;
-                            }
-                    }
-                } else if (kind == OneE::S) {
-                    size_t kk = 0;
-                    for (auto& a : ca)
-                        for (auto& bb : cb) {
-                            double sx = E[(a[0] * (pd.lb + 1) + bb[0]) * st];
-                            double sy = E[pd.esz + (a[1] * (pd.lb + 1) + bb[1]) * st];
-                            double sz = E[2 * pd.esz + (a[2] * (pd.lb + 1) + bb[2]) * st];
-                            blk[kk++] += s0 * sx * sy * sz;
-                        }
-                } else {
-                    const double beta = pp.eb;
-                    size_t kk = 0;
-                    auto s1 = [&](int d, int i, int j) -> double {
-                        if (j < 0) return 0.0;
-                        return E[d * pd.esz + (i * (pd.lb + 1) + j) * st];
-                    };
-                    for (auto& a : ca)
-                        for (auto& bb : cb) {
-                            double s[3], t[3];
-                            for (int d = 0; d < 3; ++d) {
-                                int i = a[d], j = bb[d];
-                                s[d] = s1(d, i, j);
-                                t[d] = -0.5 * (4.0 * beta * beta * s1(d, i, j + 2) - 2.0 * beta * (2 * j + 1) * s1(d, i, j) +
-                                               j * (j - 1) * s1(d, i, j - 2));
-                            }
-                            blk[kk++] += s0 * (t[0] * s[1] * s[2] + s[0] * t[1] * s[2] + s[0] * s[1] * t[2]);
-                        }
-                }
-            }
+            onee_cart(b.shells[I], b.shells[J], kind, 0, natom, Z, xyz, blk, w);
             scatter_1e(b, I, J, blk, M);
         }
     }
+}
+
+// ---------------------------------------------------------------- derivative integrals
+// d/dA_x of a primitive x^a e^{-alpha r_A^2} is 2 alpha x^{a+1} e^{..} - a x^{a-1} e^{..}.
+// For a contracted shell S of angular momentum l the derivative functions are therefore
+// combinations of the Cartesian components of two auxiliary shells on the same centre:
+// S+ (l+1, coefficients 2 alpha_k c_k) and S- (l-1, coefficients c_k), with c_k the
+// normalised coefficients of S (one normalisation per shell, shared by its components).
+struct ShiftedShells {
+    Shell plus, minus;
+    bool has_minus = false;
+};
+ShiftedShells shifted(const Shell& s) {
+    ShiftedShells r;
+    r.plus = s;
+    r.plus.l = s.l + 1;
+    r.plus.pure = false;
+    for (size_t k = 0; k < s.exps.size(); ++k) r.plus.coefs[k] = 2.0 * s.exps[k] * s.coefs[k];
+    if (s.l > 0) {
+        r.has_minus = true;
+        r.minus = s;
+        r.minus.l = s.l - 1;
+        r.minus.pure = false;
+    }
+    return r;
+}
+inline int cart_index(int l, int lx, int ly) { return (l - lx) * (l - lx + 1) / 2 + (l - lx - ly); }
+
+// Given Cartesian blocks of S+ and S- (S's index first, `rest` trailing elements each),
+// form the three derivative blocks d[dir][ncart(S)][rest] w.r.t. S's centre.
+void combine_deriv(int l, const std::vector<double>& bp, const std::vector<double>* bm, size_t rest,
+                   std::vector<double> d[3]) {
+    const auto& cs = carts().comps[l];
+    for (int dir = 0; dir < 3; ++dir) d[dir].assign(cs.size() * rest, 0.0);
+    for (size_t a = 0; a < cs.size(); ++a) {
+        const auto& c = cs[a];
+        for (int dir = 0; dir < 3; ++dir) {
+            int up[3] = {c[0], c[1], c[2]};
+            up[dir] += 1;
+            const double* sp = bp.data() + static_cast<size_t>(cart_index(l + 1, up[0], up[1])) * rest;
+            double* o = d[dir].data() + a * rest;
+            for (size_t r = 0; r < rest; ++r) o[r] = sp[r];
+            if (c[dir] > 0 && bm) {
+                int dn[3] = {c[0], c[1], c[2]};
+                dn[dir] -= 1;
+                const double* sm = bm->data() + static_cast<size_t>(cart_index(l - 1, dn[0], dn[1])) * rest;
+                const double f = static_cast<double>(c[dir]);
+                for (size_t r = 0; r < rest; ++r) o[r] -= f * sm[r];
+            }
+        }
+    }
+}
+
+// Spherical transform of a derivative block and contraction with a weight block W
+// (rows: functions of shell A at offset oa, cols: functions of shell B at offset ob).
+double contract_1e(const Basis& b, const Shell& A, const Shell& B, int oa, int ob, std::vector<double>& cart,
+                   const double* W, Work& w) {
+    w.cart = cart;
+    Shell one = dummy_s(A);
+    const Shell* s4[4] = {&A, &B, &one, &one};
+    to_spherical(s4, w);
+    const int na = A.nfunc(), nb = B.nfunc();
+    double s = 0.0;
+    for (int a = 0; a < na; ++a)
+        for (int c = 0; c < nb; ++c)
+            s += W[static_cast<size_t>(oa + a) * b.nbf + ob + c] * w.cart[static_cast<size_t>(a) * nb + c];
+    return s;
+}
+
+// g_shell[I][dir] += 2 sum_{m in I, n} W_mn d<m|O|n>/dA_I (W symmetric); for V also the
+// derivative with respect to each nucleus (translational invariance per nucleus) into g_nuc.
+void onee_grad(const Basis& b, OneE kind, const double* W, int natom, const double* Z, const double* xyz,
+               double* g_shell, double* g_nuc) {
+    const int ns = static_cast<int>(b.shells.size());
+    const int nc = kind == OneE::V ? natom : 1;
+    std::vector<double> gn(static_cast<size_t>(ns) * nc * 3, 0.0);
+#pragma omp parallel for schedule(dynamic)
+    for (int I = 0; I < ns; ++I) {
+        Work w;
+        const Shell& A = b.shells[I];
+        const ShiftedShells sh = shifted(A);
+        std::vector<double> bp, bm, d[3];
+        for (int J = 0; J < ns; ++J) {
+            const Shell& B = b.shells[J];
+            const size_t rest = static_cast<size_t>(B.ncart());
+            for (int c = 0; c < nc; ++c) {
+                const int c0 = kind == OneE::V ? c : 0, c1 = kind == OneE::V ? c + 1 : 0;
+                onee_cart(sh.plus, B, kind, c0, c1, Z, xyz, bp, w);
+                if (sh.has_minus) onee_cart(sh.minus, B, kind, c0, c1, Z, xyz, bm, w);
+                combine_deriv(A.l, bp, sh.has_minus ? &bm : nullptr, rest, d);
+                for (int dir = 0; dir < 3; ++dir)
+                    gn[(static_cast<size_t>(I) * nc + c) * 3 + dir] +=
+                        2.0 * contract_1e(b, A, B, b.offset[I], b.offset[J], d[dir], W, w);
+            }
+        }
+    }
+    for (int I = 0; I < ns; ++I)
+        for (int c = 0; c < nc; ++c)
+            for (int dir = 0; dir < 3; ++dir) {
+                const double v = gn[(static_cast<size_t>(I) * nc + c) * 3 + dir];
+                g_shell[3 * I + dir] += v;
+                if (kind == OneE::V) g_nuc[3 * c + dir] -= v;  // d/dC = -(d/dA + d/dB) for nucleus C
+            }
 }
 
 // Dipole integrals <a| r_d - O_d |b>, d = x, y, z, into M[3][nbf][nbf].  With the
@@ -671,6 +780,16 @@ void angmom_ints(const Basis& b, const double* O, double* M) {
 
 void angular_momentum(const Basis& b, const double* origin, double* M) { angmom_ints(b, origin, M); }
 void overlap(const Basis& b, double* S) { one_electron(b, OneE::S, 0, nullptr, nullptr, S); }
+void overlap_grad(const Basis& b, const double* W, double* g_shell) {
+    onee_grad(b, OneE::S, W, 0, nullptr, nullptr, g_shell, nullptr);
+}
+void kinetic_grad(const Basis& b, const double* D, double* g_shell) {
+    onee_grad(b, OneE::T, D, 0, nullptr, nullptr, g_shell, nullptr);
+}
+void potential_grad(const Basis& b, const double* D, int natom, const double* Z, const double* xyz, double* g_shell,
+                    double* g_nuc) {
+    onee_grad(b, OneE::V, D, natom, Z, xyz, g_shell, g_nuc);
+}
 void dipole(const Basis& b, const double* origin, double* M) { dipole_ints(b, origin, M); }
 void kinetic(const Basis& b, double* T) {
     one_electron(b, OneE::T, 0, nullptr, nullptr, T);
@@ -754,6 +873,124 @@ void three_index(const Basis& prim, const Basis& aux, const int64_t* dst, int64_
                 }
         }
     }
+}
+
+// sum_{P,m,n} G[P][m][n] d(P|mn)/dR over the centres R of the shells.  G is symmetric in
+// (m,n); with translational invariance d/dP = -(d/dm + d/dn), only the derivative with
+// respect to the first primitive function is needed:
+//   g_prim[M] += 2 sum_{m in M, n, P} G d_m(P|mn),   g_aux[P] -= the same sum over its P.
+void three_index_grad(const Basis& prim, const Basis& aux, const double* G, double* g_prim, double* g_aux,
+                      int nthreads) {
+    const int na = static_cast<int>(aux.shells.size());
+    const int np = static_cast<int>(prim.shells.size());
+    const int nbf = prim.nbf;
+    const size_t nn = static_cast<size_t>(nbf) * nbf;
+    std::vector<PairData> apd(na);
+    std::vector<Shell> dum(na);
+    for (int P = 0; P < na; ++P) {
+        dum[P] = dummy_s(aux.shells[P]);
+        apd[P] = make_pair(aux.shells[P], dum[P]);
+    }
+    std::vector<double> gp(static_cast<size_t>(np) * 3, 0.0), ga(static_cast<size_t>(na) * 3, 0.0);
+#pragma omp parallel num_threads(nthreads)
+    {
+        std::vector<double> lgp(gp.size(), 0.0), lga(ga.size(), 0.0);
+        Work w;
+        std::vector<double> bp, bm, d[3];
+#pragma omp for schedule(dynamic)
+        for (int ip = 0; ip < np * np; ++ip) {
+            const int M = ip / np, N = ip % np;
+            const Shell& sm = prim.shells[M];
+            const Shell& sn = prim.shells[N];
+            const ShiftedShells sh = shifted(sm);
+            PairData pplus = make_pair(sh.plus, sn);
+            PairData pminus;
+            if (sh.has_minus) pminus = make_pair(sh.minus, sn);
+            const int nm = sm.nfunc(), nnf = sn.nfunc();
+            for (int P = 0; P < na; ++P) {
+                const Shell& sp = aux.shells[P];
+                const size_t rest = static_cast<size_t>(sn.ncart()) * sp.ncart();
+                bp.resize(static_cast<size_t>(sh.plus.ncart()) * rest);
+                eri_cart(pplus, apd[P], bp.data(), w);
+                if (sh.has_minus) {
+                    bm.resize(static_cast<size_t>(sh.minus.ncart()) * rest);
+                    eri_cart(pminus, apd[P], bm.data(), w);
+                }
+                combine_deriv(sm.l, bp, sh.has_minus ? &bm : nullptr, rest, d);
+                const int nP = sp.nfunc();
+                for (int dir = 0; dir < 3; ++dir) {
+                    w.cart = d[dir];
+                    const Shell* s4[4] = {&sm, &sn, &sp, &dum[P]};
+                    to_spherical(s4, w);
+                    double v = 0.0;
+                    for (int a = 0; a < nm; ++a)
+                        for (int b = 0; b < nnf; ++b) {
+                            const size_t mn = static_cast<size_t>(prim.offset[M] + a) * nbf + prim.offset[N] + b;
+                            const double* src = w.cart.data() + (static_cast<size_t>(a) * nnf + b) * nP;
+                            for (int c = 0; c < nP; ++c) v += G[static_cast<size_t>(aux.offset[P] + c) * nn + mn] * src[c];
+                        }
+                    lgp[3 * M + dir] += 2.0 * v;
+                    lga[3 * P + dir] -= 2.0 * v;
+                }
+            }
+        }
+#pragma omp critical
+        {
+            for (size_t i = 0; i < gp.size(); ++i) gp[i] += lgp[i];
+            for (size_t i = 0; i < ga.size(); ++i) ga[i] += lga[i];
+        }
+    }
+    for (size_t i = 0; i < gp.size(); ++i) g_prim[i] += gp[i];
+    for (size_t i = 0; i < ga.size(); ++i) g_aux[i] += ga[i];
+}
+
+// sum_{P,Q} W[P][Q] d(P|Q)/dR (W symmetric): g_aux[P] += 2 sum_{p in P, Q} W_pQ d_p(p|Q).
+void metric_grad(const Basis& aux, const double* W, double* g_aux, int nthreads) {
+    const int na = static_cast<int>(aux.shells.size());
+    const int n = aux.nbf;
+    std::vector<PairData> apd(na);
+    std::vector<Shell> dum(na);
+    for (int P = 0; P < na; ++P) {
+        dum[P] = dummy_s(aux.shells[P]);
+        apd[P] = make_pair(aux.shells[P], dum[P]);
+    }
+    std::vector<double> ga(static_cast<size_t>(na) * 3, 0.0);
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+    for (int P = 0; P < na; ++P) {
+        Work w;
+        std::vector<double> bp, bm, d[3];
+        const Shell& sp = aux.shells[P];
+        const ShiftedShells sh = shifted(sp);
+        PairData pplus = make_pair(sh.plus, dum[P]);
+        PairData pminus;
+        if (sh.has_minus) pminus = make_pair(sh.minus, dum[P]);
+        double acc[3] = {0, 0, 0};
+        for (int Q = 0; Q < na; ++Q) {
+            const Shell& sq = aux.shells[Q];
+            const size_t rest = static_cast<size_t>(sq.ncart());
+            bp.resize(static_cast<size_t>(sh.plus.ncart()) * rest);
+            eri_cart(pplus, apd[Q], bp.data(), w);
+            if (sh.has_minus) {
+                bm.resize(static_cast<size_t>(sh.minus.ncart()) * rest);
+                eri_cart(pminus, apd[Q], bm.data(), w);
+            }
+            combine_deriv(sp.l, bp, sh.has_minus ? &bm : nullptr, rest, d);
+            const int nP = sp.nfunc(), nQ = sq.nfunc();
+            for (int dir = 0; dir < 3; ++dir) {
+                w.cart = d[dir];
+                const Shell* s4[4] = {&sp, &dum[P], &sq, &dum[Q]};
+                to_spherical(s4, w);
+                double v = 0.0;
+                for (int a = 0; a < nP; ++a)
+                    for (int c = 0; c < nQ; ++c)
+                        v += W[static_cast<size_t>(aux.offset[P] + a) * n + aux.offset[Q] + c] *
+                             w.cart[static_cast<size_t>(a) * nQ + c];
+                acc[dir] += 2.0 * v;
+            }
+        }
+        for (int dir = 0; dir < 3; ++dir) ga[3 * P + dir] = acc[dir];
+    }
+    for (size_t i = 0; i < ga.size(); ++i) g_aux[i] += ga[i];
 }
 
 double schwarz_diagonal(const Basis& prim, double* fun_max, double* shell_max, int nthreads) {

@@ -67,6 +67,18 @@ void three_index(const Basis& prim, const Basis& aux, const int64_t* dst, int64_
 // Schwarz diagonal (mn|mn) for all function pairs: fun_max nbf*nbf (both triangles),
 // shell_max nshell*nshell, following dfhelper.cc:322-366. Returns max value.
 double schwarz_diagonal(const Basis& prim, double* fun_max, double* shell_max, int nthreads);
+// ---- gradient contractions (derivatives with respect to shell centres) ----
+// g_shell[I][xyz] += sum_mn W_mn dS_mn/dR_I  (W symmetric), likewise T with D.
+void overlap_grad(const Basis& b, const double* W, double* g_shell);
+void kinetic_grad(const Basis& b, const double* D, double* g_shell);
+// V: basis-centre part into g_shell, operator (nuclear position) part into g_nuc[natom][3].
+void potential_grad(const Basis& b, const double* D, int natom, const double* Z, const double* xyz, double* g_shell,
+                    double* g_nuc);
+// sum_{P,m,n} G[P][m][n] d(P|mn)/dR, G symmetric in (m,n): into g_prim[nshell][3], g_aux[naux_shell][3].
+void three_index_grad(const Basis& prim, const Basis& aux, const double* G, double* g_prim, double* g_aux,
+                      int nthreads);
+// sum_{P,Q} W[P][Q] d(P|Q)/dR, W symmetric: into g_aux[naux_shell][3].
+void metric_grad(const Basis& aux, const double* W, double* g_aux, int nthreads);
 // Full (mn|ls) over a basis, nbf^4, for brute-force checks of tiny systems.
 void four_index(const Basis& b, double* out, int nthreads);
 

@@ -55,6 +55,26 @@ int p4a_ints_angular_momentum(const p4a_basis* b, const double* origin, double* 
     });
 }
 
+int p4a_ints_overlap_grad(const p4a_basis* b, const double* W, double* g_shell) {
+    return guarded([&] { p4a::overlap_grad(p4a::basis_from_desc(b), W, g_shell); });
+}
+int p4a_ints_kinetic_grad(const p4a_basis* b, const double* D, double* g_shell) {
+    return guarded([&] { p4a::kinetic_grad(p4a::basis_from_desc(b), D, g_shell); });
+}
+int p4a_ints_potential_grad(const p4a_basis* b, const double* D, int natom, const double* Z, const double* xyz,
+                            double* g_shell, double* g_nuc) {
+    return guarded([&] { p4a::potential_grad(p4a::basis_from_desc(b), D, natom, Z, xyz, g_shell, g_nuc); });
+}
+int p4a_ints_three_index_grad(const p4a_basis* prim, const p4a_basis* aux, const double* G, double* g_prim,
+                              double* g_aux, int nthreads) {
+    return guarded([&] {
+        p4a::three_index_grad(p4a::basis_from_desc(prim), p4a::basis_from_desc(aux), G, g_prim, g_aux, nthreads);
+    });
+}
+int p4a_ints_metric_grad(const p4a_basis* aux, const double* W, double* g_aux, int nthreads) {
+    return guarded([&] { p4a::metric_grad(p4a::basis_from_desc(aux), W, g_aux, nthreads); });
+}
+
 int p4a_ints_potential(const p4a_basis* b, int natom, const double* Z, const double* xyz, double* V) {
     return guarded([&] { p4a::potential(p4a::basis_from_desc(b), natom, Z, xyz, V); });
 }

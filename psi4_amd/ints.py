@@ -124,3 +124,60 @@ def boys(nmax, T):
     F = np.zeros(nmax + 1)
     _check(lib, lib.p4a_ints_boys(nmax, ctypes.c_double(T), _ptr(F)))
     return F
+
+
+# ---- gradient contractions (derivatives with respect to shell centres) ----
+def _grad_out(basis):
+    return np.zeros((basis.nshell, 3))
+
+
+def overlap_grad(basis, W):
+    """sum_mn W_mn dS_mn/dR per shell centre, [nshell, 3] (W symmetric)."""
+    lib = load_ints_lib()
+    g = _grad_out(basis)
+    cb, _keep = basis.as_c()
+    _check(lib, lib.p4a_ints_overlap_grad(ctypes.byref(cb), _ptr(np.ascontiguousarray(W, np.float64)), _ptr(g)))
+    return g
+
+
+def kinetic_grad(basis, D):
+    lib = load_ints_lib()
+    g = _grad_out(basis)
+    cb, _keep = basis.as_c()
+    _check(lib, lib.p4a_ints_kinetic_grad(ctypes.byref(cb), _ptr(np.ascontiguousarray(D, np.float64)), _ptr(g)))
+    return g
+
+
+def potential_grad(basis, mol, D):
+    """(per-shell-centre part [nshell, 3], per-nucleus part [natom, 3]) of sum_mn D_mn dV_mn/dR."""
+    lib = load_ints_lib()
+    g = _grad_out(basis)
+    gn = np.zeros((mol.natom, 3))
+    cb, _keep = basis.as_c()
+    Z = np.ascontiguousarray(mol.Z, np.float64)
+    xyz = np.ascontiguousarray(mol.xyz, np.float64)
+    _check(lib, lib.p4a_ints_potential_grad(ctypes.byref(cb), _ptr(np.ascontiguousarray(D, np.float64)), mol.natom,
+                                            _ptr(Z), _ptr(xyz), _ptr(g), _ptr(gn)))
+    return g, gn
+
+
+def three_index_grad(prim, aux, G, nthreads=None):
+    """sum_{P,mn} G[P,m,n] d(P|mn)/dR: (per primitive shell [nshell, 3], per aux shell [naux_shell, 3])."""
+    lib = load_ints_lib()
+    gp, ga = _grad_out(prim), _grad_out(aux)
+    cp, _k1 = prim.as_c()
+    ca, _k2 = aux.as_c()
+    G = np.ascontiguousarray(G, np.float64)
+    _check(lib, lib.p4a_ints_three_index_grad(ctypes.byref(cp), ctypes.byref(ca), _ptr(G), _ptr(gp), _ptr(ga),
+                                              nthreads or default_threads()))
+    return gp, ga
+
+
+def metric_grad(aux, W, nthreads=None):
+    """sum_PQ W_PQ d(P|Q)/dR per aux shell centre, [naux_shell, 3] (W symmetric)."""
+    lib = load_ints_lib()
+    ga = _grad_out(aux)
+    ca, _k = aux.as_c()
+    _check(lib, lib.p4a_ints_metric_grad(ctypes.byref(ca), _ptr(np.ascontiguousarray(W, np.float64)), _ptr(ga),
+                                         nthreads or default_threads()))
+    return ga

@@ -66,6 +66,18 @@ class Molecule:
                 e += Z[i] * Z[j] / np.linalg.norm(R[i] - R[j])
         return e
 
+    def nuclear_repulsion_energy_deriv1(self):
+        """d E_nuc / d R as [natom, 3] (libmints/molecule.cc nuclear_repulsion_energy_deriv1)."""
+        Z, R = self.Z, self.xyz
+        g = np.zeros((self.natom, 3))
+        for i in range(self.natom):
+            for j in range(i):
+                d = R[i] - R[j]
+                f = Z[i] * Z[j] * d / np.linalg.norm(d) ** 3
+                g[i] -= f
+                g[j] += f
+        return g
+
 
 def water_zmat(r_oh: float, theta_deg: float) -> Molecule:
     """H2O from the Z-matrix  O; H 1 r; H 1 r 2 theta  (Angstrom, degrees)."""
