@@ -79,6 +79,18 @@ def test_h2o_psi4_reference_energy(gpu_available):
     assert abs(res.energy - PSI4_E_H2O) < 1e-9
 
 
+def test_h2o_cc_pvtz_psi4_scf2_energy(gpu_available):
+    """f (primary) and g (fitting) functions through the device path: psi4 tests/scf2."""
+    import psi4_amd.molecule as M
+    from psi4_amd.scf import RHF
+    ang = np.array([[0.0, 0.0, -0.068516219310], [0.0, -0.790689573744, 0.543701060724],
+                    [0.0, 0.790689573744, 0.543701060724]])
+    mol = M.Molecule(["O", "H", "H"], ang / 0.52917720859)
+    announce("H2O cc-pVTZ DF-RHF on the GPU JK vs psi4 scf2")
+    res = RHF(mol, basis="cc-pvtz", e_convergence=1e-12, d_convergence=1e-10).compute_energy()
+    assert abs(res.energy - (-76.05098620307962)) < 1e-9
+
+
 def test_h2o4_scf_every_iteration(gpu_available):
     from psi4_amd.molecule import water_cluster
     mol = water_cluster((2, 2, 1), seed=0)

@@ -58,6 +58,19 @@ def test_psi4_scf_guess_read1_energy():
     assert abs(r["E"] - (-76.02663273485877)) < 1e-9   # the reference ran to d_convergence 1e-6
 
 
+def test_psi4_scf2_cc_pvtz_energy():
+    """tests/scf2: DF-RHF water / cc-pVTZ (f functions; cc-pVTZ-jkfit up to g),
+    E = -76.05098620307962 checked by psi4 to 1e-9 (input.dat:3-4, 36), E_nuc 8.8014655646.
+    psi4's oriented geometry (output.ref:85-87, Angstrom), CODATA-2006 Bohr radius."""
+    ang = np.array([[0.0, 0.0, -0.068516219310], [0.0, -0.790689573744, 0.543701060724],
+                    [0.0, 0.790689573744, 0.543701060724]])
+    mol = M.Molecule(["O", "H", "H"], ang / BOHR_2006)
+    assert abs(mol.nuclear_repulsion_energy() - 8.8014655646) < 1e-9
+    res, o = oracle_energy(mol, "cc-pvtz", "cc-pvtz-jkfit")
+    assert o.nbf == 58 and o.naux == 139
+    assert abs(res["E"] - (-76.05098620307962)) < 1e-9
+
+
 def test_psi4_jkmemory_ar5():
     """tests/pytests/test_jkmemory.py:37: MEM_DF estimate 1,590,520 doubles for Ar5/cc-pVDZ (-n1)."""
     mol = M.Molecule.from_string("Ar 0 0 0\nAr 0 0 5\nAr 0 0 15\nAr 0 0 25\nAr 0 0 35")
