@@ -60,12 +60,13 @@ def test_psi4_scf_guess_read1_energy():
 
 def test_psi4_scf2_cc_pvtz_energy():
     """tests/scf2: DF-RHF water / cc-pVTZ (f functions; cc-pVTZ-jkfit up to g),
-    E = -76.05098620307962 checked by psi4 to 1e-9 (input.dat:3-4, 36), E_nuc 8.8014655646.
-    psi4's oriented geometry (output.ref:85-87, Angstrom), CODATA-2006 Bohr radius."""
+    E = -76.05098620307962 checked by psi4 to 1e-9 (input.dat:4, 36).  psi4's oriented
+    geometry (output.ref:85-87, Angstrom) with the CODATA-2006 Bohr radius of that run
+    (E_nuc 8.801465529972067, output.ref:93)."""
     ang = np.array([[0.0, 0.0, -0.068516219310], [0.0, -0.790689573744, 0.543701060724],
                     [0.0, 0.790689573744, 0.543701060724]])
     mol = M.Molecule(["O", "H", "H"], ang / BOHR_2006)
-    assert abs(mol.nuclear_repulsion_energy() - 8.8014655646) < 1e-9
+    assert abs(mol.nuclear_repulsion_energy() - 8.801465529972067) < 1e-11
     res, o = oracle_energy(mol, "cc-pvtz", "cc-pvtz-jkfit")
     assert o.nbf == 58 and o.naux == 139
     assert abs(res["E"] - (-76.05098620307962)) < 1e-9
