@@ -221,7 +221,8 @@ def main():
     if os.path.exists(tfile):
         rec = json.load(open(tfile)).get("kernels", {}).get(kname)
         if rec:
-            per_q = (rec["read_bytes"] + rec["write_bytes"]) / rec["q_per_dispatch"]
+            # bytes read + written by all dispatches of one build, over the aux functions they cover
+            per_q = (rec["read_bytes"] + rec["write_bytes"]) / rec["q_covered"]
             q_per_launch = naux * args.steps / max(1, nlaunch)
             roofline["traffic"] = round(per_q * q_per_launch)
             roofline["traffic_source"] = f"{os.path.basename(tfile)} ({rec.get('profiled', 'rocprofv3 --pmc')})"

@@ -258,6 +258,7 @@ void build_tiles(dfjk_handle* h) {
 void upload_tiles(dfjk_handle* h) {
     hcheck(hipMemset(h->dZero.get<double>(TILE2), 0, TILE2 * sizeof(double)), "zero tile");
     if (const char* e = std::getenv("PSI4AMD_HT_ROWWAVES")) set_ht_rowwaves(std::atoi(e));
+    if (const char* e = std::getenv("PSI4AMD_HT_VARIANT")) set_ht_variant(std::atoi(e));
     const auto& sym = h->otiles_sym;
     const auto& all = h->otiles_all;
     const auto& of_sym = h->otile_of_sym;
@@ -564,15 +565,19 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             double alpha;
             size_t offl, offr;
         };
+        // padding columns of X are zero: each pass stops at the contraction's k-step after
+        // the last real column of its group
+        const int kstep = prec == DFJK_PREC_BF16X3 ? 32 : 16;
         std::vector<Pass> passes;
         if (signed_sym) {
-            if (kplus > 0) passes.push_back({0, kplus, 1.0, 0, 0});
-            passes.push_back({kplus, ncol_eff, -1.0, 0, 0});
+            if (!plus.empty()) passes.push_back({0, round_up(static_cast<int>(plus.size()), kstep), 1.0, 0, 0});
+            passes.push_back({kplus, kplus + round_up(static_cast<int>(minus.size()), kstep), -1.0, 0, 0});
         } else if (cross) {
-            passes.push_back({0, wcross, 1.0, 0, static_cast<size_t>(wcross)});
-            passes.push_back({0, wcross, 1.0, static_cast<size_t>(wcross), 0});
+            const int kc = round_up(half, kstep);
+            passes.push_back({0, kc, 1.0, 0, static_cast<size_t>(wcross)});
+            passes.push_back({0, kc, 1.0, static_cast<size_t>(wcross), 0});
         } else {
-            passes.push_back({0, ldc, 1.0, 0, 0});
+            passes.push_back({0, std::min(ldc, round_up(ncol, kstep)), 1.0, 0, 0});
         }
 
         const float *B32 = nullptr, *c32l = nullptr, *c32r = nullptr;

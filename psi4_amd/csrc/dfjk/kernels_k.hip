@@ -266,6 +266,154 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 }
 
 // =====================================================================================
+// Half-transform, wide-wave shape (FP64 / FP32, 16x16x4 MFMA): 256 threads = 4 waves at
+// one wave per SIMD, each wave all 32*RT rows x 16*NBW columns (2*RT*NBW accumulator
+// tiles, held in the accumulation registers), workgroup tile 32*RT x 64*NBW, K-step 16:
+// 2*RT*NBW*4 MFMAs per wave between barriers and (2*RT + NBW) fragment reads per
+// 2*RT*NBW MFMAs.  Rows at or beyond xrows are not written (the last row block may
+// extend past the X buffer when 32*RT does not divide it).
+// grid = ceil(nT / RT) row blocks x nq x npanel in the XCD-aware flat order.
+// =====================================================================================
+template <typename T, int NBW, int RT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_ht_wide(
+    const T* __restrict__ B, const int* __restrict__ tidx, int nT, int ntp, int q0, int nq, int npanel,
+    const T* __restrict__ C, int ldc, T* __restrict__ X, int xrows, const T* __restrict__ ztile) {
+    constexpr bool F64 = sizeof(T) == 8;
+    constexpr int NTHR = 256, ROWS = 32 * RT, NI = 2 * RT;
+    constexpr int PAN = 64 * NBW;
+    constexpr int KS = 16;
+    constexpr int AP = KS + 1;
+    constexpr int CP = PAN + 4;
+    constexpr int NA = ROWS * KS / NTHR;  // A elements per thread per step
+    constexpr int NC = KS * PAN / NTHR;   // C elements per thread per step
+    static_assert((ROWS * KS) % NTHR == 0 && (KS * PAN) % NTHR == 0, "staging shape");
+    __shared__ T As[2][ROWS * AP];
+    __shared__ T Cs[2][KS * CP];
+    __shared__ StepList<RT> sl;
+    typedef typename std::conditional<F64, d4_t, f4_t>::type acc_t;
+
+    const int tid = threadIdx.x, lane = tid & 63, wn = tid >> 6;
+    const int nrb = (nT + RT - 1) / RT;
+    const XcdItem xi = xcd_map(blockIdx.x, nrb * npanel, nq);
+    if (!xi.valid) return;
+    const int rb = xi.item / npanel;
+    const int I0 = RT * rb;
+    const int q = xi.group;
+    const int col0 = (xi.item % npanel) * PAN;
+    const T* BQ = B + static_cast<size_t>(q0 + q) * ntp * TILE2;
+
+    build_steps<RT>(sl, tidx, nT, I0, tid, NTHR);
+    const int nsteps = sl.n * (TILE / KS);
+
+    acc_t acc[NI][NBW];
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    T ra[NA], rc[NC];
+    int lJ = 0;
+    auto gload = [&](int st) {
+        const int J = sl.J[st / (TILE / KS)];
+        lJ = J;
+        const int kh = (st % (TILE / KS)) * KS;
+#pragma unroll
+        for (int it = 0; it < NA; ++it) {
+            const ACoord ac = a_coord<KS>(tid + NTHR * it, I0, J);
+            const int off = sl.off[J * RT + ac.sub];
+            const T* tp = off >= 0 ? BQ + off : ztile;
+            ra[it] = (I0 + ac.sub <= J) ? tp[ac.r * TILE + kh + ac.c] : tp[(kh + ac.c) * TILE + ac.r];
+        }
+        const T* Cj = C + static_cast<size_t>(J * TILE + kh) * ldc + col0;
+#pragma unroll
+        for (int it = 0; it < NC; ++it) {
+            const int e = tid + NTHR * it;
+            const int kk = e / PAN, cc = e - kk * PAN;
+            rc[it] = Cj[static_cast<size_t>(kk) * ldc + cc];
+        }
+    };
+    auto lstore = [&](int buf) {
+        const int J = lJ;
+#pragma unroll
+        for (int it = 0; it < NA; ++it) {
+            const ACoord ac = a_coord<KS>(tid + NTHR * it, I0, J);
+            As[buf][(ac.sub * 32 + ac.r) * AP + ac.c] = ra[it];
+        }
+#pragma unroll
+        for (int it = 0; it < NC; ++it) {
+            const int e = tid + NTHR * it;
+            const int kk = e / PAN, cc = e - kk * PAN;
+            Cs[buf][kk * CP + cc] = rc[it];
+        }
+    };
+    // Fragments of one k4 sub-step are loaded while the MFMAs of the previous one run
+    // (explicit double buffer; the scheduling barriers keep the compiler from hoisting
+    // every sub-step's loads, which would not fit in the register file).
+    auto frag = [&](int buf, int ks, T* a, T* b) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) a[i] = As[buf][(i * 16 + (lane & 15)) * AP + ks + (lane >> 4)];
+#pragma unroll
+        for (int jj = 0; jj < NBW; ++jj)
+            b[jj] = Cs[buf][(ks + (lane >> 4)) * CP + wn * NBW * 16 + jj * 16 + (lane & 15)];
+    };
+    auto mfma = [&](const T* a, const T* b) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int jj = 0; jj < NBW; ++jj) {
+                if constexpr (F64)
+                    acc[i][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[jj], acc[i][jj], 0, 0, 0);
+                else
+                    acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[jj], acc[i][jj], 0, 0, 0);
+            }
+    };
+    auto compute = [&](int buf) {
+        T a0[NI], b0[NBW], a1[NI], b1[NBW];
+        frag(buf, 0, a0, b0);
+        frag(buf, 4, a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma(a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        frag(buf, 8, a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma(a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        frag(buf, 12, a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma(a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma(a1, b1);
+    };
+
+    if (nsteps > 0) {
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        for (int st = 0; st < nsteps; ++st) {
+            const bool more = st + 1 < nsteps;
+            if (more) gload(st + 1);
+            compute(buf);
+            if (more) lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    T* Xq = X + static_cast<size_t>(q) * xrows * ldc;
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = F64 ? (4 * r + (lane >> 4)) : (4 * (lane >> 4) + r);
+                const int m = ROWS * rb + i * 16 + rr;
+                if (m < xrows)
+                    Xq[static_cast<size_t>(m) * ldc + col0 + wn * NBW * 16 + j * 16 + (lane & 15)] = acc[i][j][r];
+            }
+}
+
+// =====================================================================================
 // Half-transform, 3-term BF16 split: X = Bh Ch + Bh Cl + Bl Ch with FP32 accumulation.
 // B as hi/lo BF16 copies, C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one
 // column tile (32).  WM*256 threads, workgroup tile 64*WM rows x 64*NBW columns.
@@ -784,7 +932,21 @@ static void ht_mfma_wm(const TileGeom& g, const T* B, int q0, int nq, const T* C
     }
 }
 
-int g_ht_wm = 0;  // 0: default shape; 1 or 2 force the row-wave count (tuning sweeps)
+int g_ht_wm = 0;       // 0: default shape; 1 or 2 force the row-wave count (tuning sweeps)
+int g_ht_variant = 0;  // 0: two waves per SIMD (k_ht_mfma); 1: wide waves (k_ht_wide)
+
+// FP64 accumulators of 2*RT*NBW 16x16 tiles must fit the 256 accumulation registers
+// (RT = 3 at NBW = 5); FP32 takes four row tiles.
+template <typename T, int NBW>
+static void ht_wide_nbw(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int npanel, T* X,
+                        int xrows, const void* ztile, hipStream_t s) {
+    constexpr int RT = sizeof(T) == 8 ? (NBW == 5 ? 3 : 4) : 4;
+    const int nrb = (g.nT + RT - 1) / RT;
+    require(static_cast<size_t>(nrb) * RT * 32 >= static_cast<size_t>(g.nbfp), "ht_wide rows");
+    dim3 grid(xcd_grid(nq, nrb * npanel));
+    hipLaunchKernelGGL((k_ht_wide<T, NBW, RT>), grid, dim3(256), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, C,
+                       ldc, X, xrows, static_cast<const T*>(ztile));
+}
 
 template <typename T>
 static void ht_mfma(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int nbw, int npanel, T* X,
@@ -792,6 +954,14 @@ static void ht_mfma(const TileGeom& g, const T* B, int q0, int nq, const T* C, i
     require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 &&
                 xrows == (g.nT + 3) / 4 * 128 && g.nT <= kMaxColTiles && zt != nullptr,
             "ht_mfma shape");
+    if (g_ht_variant == 1 && nbw >= 3) {
+        switch (nbw) {
+            case 3: ht_wide_nbw<T, 3>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
+            case 4: ht_wide_nbw<T, 4>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
+            default: ht_wide_nbw<T, 5>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
+        }
+        return;
+    }
     const int wm = g_ht_wm ? g_ht_wm : 2;
     if (wm == 1)
         ht_mfma_wm<T, 1>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
@@ -808,6 +978,7 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
     ht_mfma<float>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
 }
 void set_ht_rowwaves(int wm) { g_ht_wm = (wm == 1 || wm == 2) ? wm : 0; }
+void set_ht_variant(int v) { g_ht_variant = (v == 1) ? 1 : 0; }
 
 template <int WM>
 static void ht_bf3_wm(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,

@@ -21,6 +21,8 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
                     float* X, int xrows, const void* zt, hipStream_t s);
 // Row waves per half-transform workgroup: 1 (256 threads) or 2 (512 threads); 0 = default.
 void set_ht_rowwaves(int wm);
+// Half-transform shape for FP64/FP32 panels of 3-5 column blocks: 0 two waves per SIMD, 1 wide waves.
+void set_ht_variant(int v);
 // 3-term BF16: B hi/lo [naux][ntp][32][32], C^T hi/lo [ldc][nbfp], X hi/lo [nq][xrows][ldc].
 void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
                    const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
