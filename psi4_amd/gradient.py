@@ -37,6 +37,49 @@ def _schwarz_pair_mask(basis, cutoff=1e-12):
     return fm >= cutoff * cutoff / mx
 
 
+def _device_adjoint(jk):
+    naux = jk.info()["naux"]
+    jk.set_aux_range(0, naux)
+    B = jk.get_B_dense(0, naux)
+
+    def adj(Co):
+        E2, _, Gam = jk.energy_adjoint(np.ascontiguousarray(Co), True)
+        return E2, Gam, B
+    return adj
+
+
+def _one_electron_terms(mol, bs, Dt, Wt):
+    natom = mol.natom
+    gT = ints.kinetic_grad(bs, Dt)
+    gV, gVn = ints.potential_grad(bs, mol, Dt)
+    return {"Nuclear": mol.nuclear_repulsion_energy_deriv1(),
+            "Core": _per_atom(gT + gV, bs, natom) + gVn,
+            "Overlap": -_per_atom(ints.overlap_grad(bs, Wt), bs, natom)}
+
+
+def _jk_term(mol, bs, aux, Gam, coef_fit, L, nthreads):
+    """Contract dE/dB (Gamma, fitted basis) and dE/dJ (given in the fitted basis as
+    M^-T (dE/dJ) M^-1 = coef_fit) with the derivative integrals."""
+    naux, nbf = Gam.shape[0], Gam.shape[1]
+    G = solve_triangular(L, Gam.reshape(naux, -1), lower=True, trans="T").reshape(naux, nbf, nbf)
+    G *= _schwarz_pair_mask(bs)[None, :, :]
+    gp, ga = ints.three_index_grad(bs, aux, G, nthreads)
+    del G
+    # dE/dJ = M^T coef_fit M with M = L^-1
+    T = solve_triangular(L, coef_fit, lower=True, trans="T")
+    coef = solve_triangular(L, T.T, lower=True, trans="T").T
+    coef = 0.5 * (coef + coef.T)
+    ga = ga + ints.metric_grad(aux, coef, nthreads)
+    return _per_atom(gp, bs, mol.natom) + _per_atom(ga, aux, mol.natom)
+
+
+def _fitted_d_and_w(B, Co):
+    D = Co @ Co.T
+    dq = np.einsum("qmn,mn->q", B, D, optimize=True)
+    Y = np.einsum("mi,qmn,nj->qij", Co, B, Co, optimize=True).reshape(B.shape[0], -1)
+    return dq, Y @ Y.T
+
+
 def rhf_gradient(wfn, adjoint=None, nthreads=None):
     """Nuclear gradient [natom, 3] of a converged ``psi4_amd.scf.RHF``.
 
@@ -47,45 +90,49 @@ def rhf_gradient(wfn, adjoint=None, nthreads=None):
     if wfn.Ca is None:
         raise RuntimeError("RHF: compute_energy() must run before the gradient")
     mol, bs, aux = wfn.mol, wfn.basis, wfn.aux
-    nocc = wfn.nocc
-    Co = np.ascontiguousarray(wfn.Ca[:, :nocc])
-    eps = wfn.eps[:nocc]
-    D = Co @ Co.T
-    Dt = 2.0 * D
-    Wt = 2.0 * (Co * eps) @ Co.T
-    natom = mol.natom
-    terms = {}
-    terms["Nuclear"] = mol.nuclear_repulsion_energy_deriv1()
-    gT = ints.kinetic_grad(bs, Dt)
-    gV, gVn = ints.potential_grad(bs, mol, Dt)
-    terms["Core"] = _per_atom(gT + gV, bs, natom) + gVn
-    terms["Overlap"] = -_per_atom(ints.overlap_grad(bs, Wt), bs, natom)
-
-    if adjoint is None:
-        jk = wfn.jk()
-        naux = jk.info()["naux"]
-        jk.set_aux_range(0, naux)
-        E2, _, Gam = jk.energy_adjoint(Co, True)
-        B = jk.get_B_dense(0, naux)
-    else:
-        E2, Gam, B = adjoint(Co)
-    naux, nbf = B.shape[0], B.shape[1]
+    Co = np.ascontiguousarray(wfn.Ca[:, :wfn.nocc])
+    eps = wfn.eps[:wfn.nocc]
+    terms = _one_electron_terms(mol, bs, 2.0 * Co @ Co.T, 2.0 * (Co * eps) @ Co.T)
+    adjoint = adjoint or _device_adjoint(wfn.jk())
+    E2, Gam, B = adjoint(Co)
     L = cholesky(ints.metric(aux), lower=True)
-    # dE/d(P|mn) = M^T Gamma = L^-T Gamma, restricted to the pairs the engine keeps
-    G = solve_triangular(L, Gam.reshape(naux, -1), lower=True, trans="T").reshape(naux, nbf, nbf)
-    G *= _schwarz_pair_mask(bs)[None, :, :]
-    gp, ga = ints.three_index_grad(bs, aux, G, nthreads)
-    del G
-    # dE/d(P|Q) = -2 d d^T + M^T W M
-    dq = np.einsum("qmn,mn->q", B, D, optimize=True)
-    Y = np.einsum("mi,qmn,nj->qij", Co, B, Co, optimize=True).reshape(naux, -1)
-    Wq = Y @ Y.T
-    d = solve_triangular(L, dq, lower=True, trans="T")
-    T = solve_triangular(L, Wq, lower=True, trans="T")
-    MtWM = solve_triangular(L, T.T, lower=True, trans="T").T
-    coef = -2.0 * np.outer(d, d) + 0.5 * (MtWM + MtWM.T)
-    ga = ga + ints.metric_grad(aux, coef, nthreads)
-    terms["JK"] = _per_atom(gp, bs, natom) + _per_atom(ga, aux, natom)
+    # dE/d(P|Q) = -2 d d^T + M^T W M  (fitted: -2 d' d'^T + W)
+    dq, Wq = _fitted_d_and_w(B, Co)
+    terms["JK"] = _jk_term(mol, bs, aux, Gam, -2.0 * np.outer(dq, dq) + Wq, L, nthreads)
     terms["E_2"] = E2
+    g = terms["Nuclear"] + terms["Core"] + terms["Overlap"] + terms["JK"]
+    return g, terms
+
+
+def uhf_gradient(wfn, adjoint=None, nthreads=None):
+    """Nuclear gradient of a converged ``psi4_amd.uhf.UHF`` (two spin densities).
+
+    E_2 = 1/2 tr(D_t J(D_t)) - 1/2 sum_s tr(D_s K(D_s)) = 1/2 [E_R(C_a) + E_R(C_b)]
+    - 1/2 tr(D_s J(D_s)) with E_R(C) = tr(D(2J - K)) the closed-shell form whose adjoint
+    the engine provides and D_s = D_a - D_b, so
+      dE_2/dB_Q = 1/2 [Gamma(C_a) + Gamma(C_b)]_Q - d'_Q(D_s) D_s,
+      dE_2/dJ  (fitted) = -1/2 d'_t d'_t^T + 1/2 (W_a + W_b).
+    """
+    if wfn.Ca is None:
+        raise RuntimeError("UHF: compute_energy() must run before the gradient")
+    mol, bs, aux = wfn.mol, wfn.basis, wfn.aux
+    Ca = np.ascontiguousarray(wfn.Ca[:, :wfn.nalpha])
+    Cb = np.ascontiguousarray(wfn.Cb[:, :wfn.nbeta])
+    Da, Db = Ca @ Ca.T, Cb @ Cb.T
+    Wt = (Ca * wfn.eps_a[:wfn.nalpha]) @ Ca.T + (Cb * wfn.eps_b[:wfn.nbeta]) @ Cb.T
+    terms = _one_electron_terms(mol, bs, Da + Db, Wt)
+    adjoint = adjoint or _device_adjoint(wfn.jk())
+    Ea, Ga, B = adjoint(Ca)
+    Eb, Gb, _ = adjoint(Cb)
+    dqa, Wa = _fitted_d_and_w(B, Ca)
+    dqb, Wb = _fitted_d_and_w(B, Cb)
+    Ds = Da - Db
+    dqs = dqa - dqb
+    Gam = 0.5 * (Ga + Gb) - dqs[:, None, None] * Ds[None, :, :]
+    del Ga, Gb
+    L = cholesky(ints.metric(aux), lower=True)
+    dqt = dqa + dqb
+    terms["JK"] = _jk_term(mol, bs, aux, Gam, -0.5 * np.outer(dqt, dqt) + 0.5 * (Wa + Wb), L, nthreads)
+    terms["E_2"] = 0.5 * (Ea + Eb) - 0.5 * float(dqs @ dqs)
     g = terms["Nuclear"] + terms["Core"] + terms["Overlap"] + terms["JK"]
     return g, terms

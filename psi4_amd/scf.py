@@ -10,9 +10,10 @@ not re-initialized, scf_iterator.py:153-159).
 Incremental mixed-precision Fock builds (``mixed_precision``): after a full
 FP64 build, each iteration adds J(dD), K(dD) of the density change to FP64 J
 and K, with K(dD) in FP32 or 3-term BF16 on the device.  dD is passed in the
-stable form dD = C_new dC^T + dC C_old^T (C_left=[C_new, dC], C_right=[dC,
-C_old], C_old aligned to C_new by an orthogonal Procrustes rotation), so every
-low-precision product carries an O(|dC|) factor.  A full FP64 rebuild runs
+cross form dD = A V^T + V A^T (C_left=[A, V], C_right=[V, A], A = (C_new +
+C_old)/2, V = C_new - C_old, C_old aligned to C_new by an orthogonal Procrustes
+rotation), so every low-precision product carries the small factor V and K(dD)
+stays symmetric.  A full FP64 rebuild runs
 every ``incfock_full_every`` iterations (psi4's INCFOCK_FULL_FOCK_EVERY
 pattern, libfock/DirectJK.cc:333-349), and once the orbital gradient falls
 below ``mixed_until`` every remaining iteration is a full FP64 build.

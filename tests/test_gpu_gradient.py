@@ -56,3 +56,24 @@ def test_h2o4_gradient_device_vs_oracle_adjoint(gpu_available):
     assert abs(terms["E_2"] - to["E_2"]) < 1e-11 * abs(to["E_2"])
     assert np.abs(g.sum(axis=0)).max() < 1e-9
     assert np.abs(np.cross(mol.xyz, g).sum(axis=0)).max() < 1e-8
+
+
+def test_scf6_uhf_energy_and_gradient_on_gpu(gpu_available):
+    """Triplet DF-UHF (two density pairs per JK call) and its gradient (two device adjoints)
+    against psi4 scf6 (output.ref:6766, 6876-6882)."""
+    import psi4_amd.molecule as M
+    from psi4_amd.gradient import uhf_gradient
+    from psi4_amd.scf import RHF
+    from psi4_amd.uhf import UHF
+    xyz = ANG / 0.52917720859
+    announce("scf6 triplet water DF-UHF energy and gradient on the GPU JK")
+    guess = RHF(M.Molecule(["O", "H", "H"], xyz)).compute_energy().C
+    wfn = UHF(M.Molecule(["O", "H", "H"], xyz, multiplicity=3), e_convergence=1e-12, d_convergence=1e-10,
+              guess=guess)
+    res = wfn.compute_energy()
+    g, _ = uhf_gradient(wfn)
+    ref = np.array([[0.0, 0.0, 0.070328030317], [0.0, 0.046340283191, -0.035164015159],
+                    [0.0, -0.046340283191, -0.035164015159]])
+    print(f"  E {res.energy:.12f}  max |g - psi4| {np.abs(g - ref).max():.2e}", flush=True)
+    assert abs(res.energy - (-75.79200445921231)) < 1e-9
+    assert np.abs(g - ref).max() < 2e-8
