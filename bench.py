@@ -45,7 +45,81 @@ def parse():
     ap.add_argument("--skip-fp64", action="store_true", help="do not time the FP64 reference build")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device copy of this many GiB (HBM counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--scf-cluster", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
+                    help="instead of the synthetic build: RHF on a seeded water cluster of NX*NY*NZ molecules, "
+                         "plain FP64 vs incremental mixed precision, then the energy adjoint")
+    ap.add_argument("--cpu-check", action="store_true", help="with --scf-cluster: also run the CPU oracle RHF")
     return ap.parse_args()
+
+
+def scf_workload(args):
+    """RHF on a seeded water cluster (cc-pVDZ / cc-pVDZ-jkfit): plain FP64 GPU JK vs the incremental
+    mixed-precision builds with FP64 finishing iterations, then dE/dC and dE/dB on the device."""
+    import numpy as np
+    import torch
+    from psi4_amd.molecule import water_cluster
+    from psi4_amd.scf import RHF
+    nx, ny, nz = args.scf_cluster
+    mol = water_cluster((nx, ny, nz), seed=0)
+    prec = args.precision if args.precision != "fp64" else "bf16x3"
+    log(f"(H2O)_{nx * ny * nz}: setup (host integrals, device fit)")
+    t0 = time.perf_counter()
+    plain = RHF(mol)
+    jk = plain.jk()
+    t_setup = time.perf_counter() - t0
+    info = jk.info()
+    log(f"nbf {info['nbf']} naux {info['naux']}, setup {t_setup:.1f} s; plain FP64 RHF")
+    t0 = time.perf_counter()
+    rp = plain.compute_energy()
+    t_plain = time.perf_counter() - t0
+    log(f"plain: E {rp.energy:.10f} in {rp.iterations} iterations, {t_plain:.2f} s")
+    mixed = RHF(mol, jk=jk, mixed_precision=prec)
+    t0 = time.perf_counter()
+    rm = mixed.compute_energy()
+    t_mixed = time.perf_counter() - t0
+    kinds = [h["build"] for h in rm.history]
+    log(f"mixed ({prec}): E {rm.energy:.10f} in {rm.iterations} iterations ({kinds.count(prec)} mixed builds), "
+        f"{t_mixed:.2f} s")
+    # adjoint on the device: E, dE/dC, dE/dB; check <B, dE/dB> = 2E and <C, dE/dC> = 4E
+    C = torch.from_numpy(np.ascontiguousarray(rm.C[:, :mixed.nocc])).cuda()
+    nbf, naux = info["nbf"], info["naux"]
+    dC = torch.empty_like(C)
+    dB = torch.empty((naux, nbf, nbf), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E2 = jk.energy_adjoint_device(C.data_ptr(), C.shape[1], dC.data_ptr(), dB.data_ptr())
+    t_adj = time.perf_counter() - t0
+    Bd = torch.from_numpy(jk.get_B_dense(0, naux)).cuda()
+    euler_B = float((Bd * dB).sum()) / (2 * E2) - 1.0
+    euler_C = float((C * dC).sum()) / (4 * E2) - 1.0
+    del Bd, dB
+    out = {"workload": f"RHF (H2O)_{nx * ny * nz} cc-pVDZ/cc-pVDZ-jkfit, seeded cluster",
+           "nbf": nbf, "naux": naux, "nocc": int(mixed.nocc), "setup_s": round(t_setup, 2),
+           "plain_fp64": {"E": rp.energy, "iterations": rp.iterations, "wall_s": round(t_plain, 3),
+                          "jk_s": round(rp.timings["jk_s"], 3)},
+           "mixed": {"precision": prec, "E": rm.energy, "iterations": rm.iterations,
+                     "mixed_builds": kinds.count(prec), "wall_s": round(t_mixed, 3),
+                     "jk_s": round(rm.timings["jk_s"], 3)},
+           "dE_mixed_vs_plain": abs(rm.energy - rp.energy),
+           "adjoint": {"wall_s": round(t_adj, 3), "E_2": E2, "rel_<B,dE/dB>/2E-1": euler_B,
+                       "rel_<C,dE/dC>/4E-1": euler_C}}
+    if args.cpu_check:
+        from oracle.dfhelper import MemDFJKOracle
+        from oracle.rhf import rhf
+        from psi4_amd import ints
+        log("CPU oracle RHF (MemDFJK restatement, Cholesky fit)")
+        t0 = time.perf_counter()
+        ora = MemDFJKOracle(plain.basis, plain.aux, fit="cholesky")
+        S = ints.overlap(plain.basis)
+        H = ints.kinetic(plain.basis) + ints.potential(plain.basis, mol)
+
+        def jkf(Cx):
+            _, Jo, Ko = ora.compute([Cx])
+            return Jo[0], Ko[0]
+        ro = rhf(H, S, mol.nelectron() // 2, mol.nuclear_repulsion_energy(), jkf)
+        out["cpu_oracle"] = {"E": ro["E"], "wall_s": round(time.perf_counter() - t0, 1),
+                             "dE_mixed_vs_cpu": abs(rm.energy - ro["E"]), "dE_plain_vs_cpu": abs(rp.energy - ro["E"])}
+    print(json.dumps(out), flush=True)
 
 
 def log(msg):
@@ -62,6 +136,9 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.scf_cluster:
+        return scf_workload(args)
 
     if args.copy_probe > 0:
         n = int(args.copy_probe * (1 << 30)) // 8

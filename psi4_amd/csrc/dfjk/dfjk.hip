@@ -170,7 +170,7 @@ struct dfjk_handle {
     std::vector<int> otile_of_sym, otile_of_all;
     int ntile_sym = 0, ntile_all = 0;
     // resident tensor and its lower-precision copies
-    DevBuf dB, dB32, dBh, dBl;
+    DevBuf dB, dB32, dBh, dBl, dBhT, dBlT;
     DevBuf dZero;  // one zero TILE x TILE FP64 block, read in place of absent blocks
     // workspace
     DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G, ws_sign, ws_c32, ws_cbf, ws_cmap;
@@ -258,7 +258,6 @@ void build_tiles(dfjk_handle* h) {
 void upload_tiles(dfjk_handle* h) {
     hcheck(hipMemset(h->dZero.get<double>(TILE2), 0, TILE2 * sizeof(double)), "zero tile");
     if (const char* e = std::getenv("PSI4AMD_HT_ROWWAVES")) set_ht_rowwaves(std::atoi(e));
-    if (const char* e = std::getenv("PSI4AMD_HT_VARIANT")) set_ht_variant(std::atoi(e));
     const auto& sym = h->otiles_sym;
     const auto& all = h->otiles_all;
     const auto& of_sym = h->otile_of_sym;
@@ -398,10 +397,20 @@ void ensure_Bbf(dfjk_handle* h) {
         launch_to_bf_split(h->dB.as<double>(), hi, lo, b_doubles(h), h->stream);
     }
 }
+// tile-transposed BF16 pair, read by the half-transform for blocks used as (J, I)
+void ensure_BbfT(dfjk_handle* h) {
+    if (!h->dBhT.p) {
+        uint16_t* hi = h->dBhT.get<uint16_t>(b_doubles(h));
+        uint16_t* lo = h->dBlT.get<uint16_t>(b_doubles(h));
+        launch_to_bf_split_t(h->dB.as<double>(), hi, lo, b_doubles(h), h->stream);
+    }
+}
 void invalidate_copies(dfjk_handle* h) {
     h->dB32.release();
     h->dBh.release();
     h->dBl.release();
+    h->dBhT.release();
+    h->dBlT.release();
 }
 
 // One density pair: Cl/Cr are device [nbf][ncol] row-major (ld = ncol).
@@ -594,6 +603,7 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             }
         } else if (prec == DFJK_PREC_BF16X3) {
             ensure_Bbf(h);
+            ensure_BbfT(h);
             uint16_t* a = h->ws_cbf.get<uint16_t>(4 * csz);
             cth_l = a;
             ctl_l = a + csz;
@@ -627,10 +637,12 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             } else {
                 uint16_t* xl = reinterpret_cast<uint16_t*>(XL);
                 uint16_t* xr = reinterpret_cast<uint16_t*>(XR);
-                launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), qa, nq, cth_l, ctl_l, ldc, nbw, npanel,
+                launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), h->dBhT.as<uint16_t>(),
+                              h->dBlT.as<uint16_t>(), qa, nq, cth_l, ctl_l, ldc, nbw, npanel,
                               xl, xl + xq * qblk, xrows, h->dZero.p, s);
                 if (general)
-                    launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), qa, nq, cth_r, ctl_r, ldc, nbw,
+                    launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), h->dBhT.as<uint16_t>(),
+                                  h->dBlT.as<uint16_t>(), qa, nq, cth_r, ctl_r, ldc, nbw,
                                   npanel, xr, xr + xq * qblk, xrows, h->dZero.p, s);
             }
             tm.end(T_HT, e0);
@@ -643,17 +655,17 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                     double* xl = reinterpret_cast<double*>(XL) + pp.offl;
                     double* xr = reinterpret_cast<double*>(XR) + pp.offr;
                     launch_kgemm2_f64(xl, xr, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha,
-                                      acc, s);
+                                      acc, upper ? 1 : 0, s);
                 } else if (prec == DFJK_PREC_FP32) {
                     float* xl = reinterpret_cast<float*>(XL) + pp.offl;
                     float* xr = reinterpret_cast<float*>(XR) + pp.offr;
                     launch_kgemm2_f32(xl, xr, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha,
-                                      acc, s);
+                                      acc, upper ? 1 : 0, s);
                 } else {
                     uint16_t* xl = reinterpret_cast<uint16_t*>(XL) + pp.offl;
                     uint16_t* xr = reinterpret_cast<uint16_t*>(XR) + pp.offr;
                     launch_kgemm_bf3(xl, xl + xq * qblk, xr, xr + xq * qblk, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp,
-                                     otiles, ntile, P, pp.alpha, acc, s);
+                                     otiles, ntile, P, pp.alpha, acc, upper ? 1 : 0, s);
                 }
                 h->n_kgemm += 1;
             }

@@ -266,154 +266,6 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 }
 
 // =====================================================================================
-// Half-transform, wide-wave shape (FP64 / FP32, 16x16x4 MFMA): 256 threads = 4 waves at
-// one wave per SIMD, each wave all 32*RT rows x 16*NBW columns (2*RT*NBW accumulator
-// tiles, held in the accumulation registers), workgroup tile 32*RT x 64*NBW, K-step 16:
-// 2*RT*NBW*4 MFMAs per wave between barriers and (2*RT + NBW) fragment reads per
-// 2*RT*NBW MFMAs.  Rows at or beyond xrows are not written (the last row block may
-// extend past the X buffer when 32*RT does not divide it).
-// grid = ceil(nT / RT) row blocks x nq x npanel in the XCD-aware flat order.
-// =====================================================================================
-template <typename T, int NBW, int RT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_ht_wide(
-    const T* __restrict__ B, const int* __restrict__ tidx, int nT, int ntp, int q0, int nq, int npanel,
-    const T* __restrict__ C, int ldc, T* __restrict__ X, int xrows, const T* __restrict__ ztile) {
-    constexpr bool F64 = sizeof(T) == 8;
-    constexpr int NTHR = 256, ROWS = 32 * RT, NI = 2 * RT;
-    constexpr int PAN = 64 * NBW;
-    constexpr int KS = 16;
-    constexpr int AP = KS + 1;
-    constexpr int CP = PAN + 4;
-    constexpr int NA = ROWS * KS / NTHR;  // A elements per thread per step
-    constexpr int NC = KS * PAN / NTHR;   // C elements per thread per step
-    static_assert((ROWS * KS) % NTHR == 0 && (KS * PAN) % NTHR == 0, "staging shape");
-    __shared__ T As[2][ROWS * AP];
-    __shared__ T Cs[2][KS * CP];
-    __shared__ StepList<RT> sl;
-    typedef typename std::conditional<F64, d4_t, f4_t>::type acc_t;
-
-    const int tid = threadIdx.x, lane = tid & 63, wn = tid >> 6;
-    const int nrb = (nT + RT - 1) / RT;
-    const XcdItem xi = xcd_map(blockIdx.x, nrb * npanel, nq);
-    if (!xi.valid) return;
-    const int rb = xi.item / npanel;
-    const int I0 = RT * rb;
-    const int q = xi.group;
-    const int col0 = (xi.item % npanel) * PAN;
-    const T* BQ = B + static_cast<size_t>(q0 + q) * ntp * TILE2;
-
-    build_steps<RT>(sl, tidx, nT, I0, tid, NTHR);
-    const int nsteps = sl.n * (TILE / KS);
-
-    acc_t acc[NI][NBW];
-#pragma unroll
-    for (int i = 0; i < NI; ++i)
-#pragma unroll
-        for (int j = 0; j < NBW; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
-    T ra[NA], rc[NC];
-    int lJ = 0;
-    auto gload = [&](int st) {
-        const int J = sl.J[st / (TILE / KS)];
-        lJ = J;
-        const int kh = (st % (TILE / KS)) * KS;
-#pragma unroll
-        for (int it = 0; it < NA; ++it) {
-            const ACoord ac = a_coord<KS>(tid + NTHR * it, I0, J);
-            const int off = sl.off[J * RT + ac.sub];
-            const T* tp = off >= 0 ? BQ + off : ztile;
-            ra[it] = (I0 + ac.sub <= J) ? tp[ac.r * TILE + kh + ac.c] : tp[(kh + ac.c) * TILE + ac.r];
-        }
-        const T* Cj = C + static_cast<size_t>(J * TILE + kh) * ldc + col0;
-#pragma unroll
-        for (int it = 0; it < NC; ++it) {
-            const int e = tid + NTHR * it;
-            const int kk = e / PAN, cc = e - kk * PAN;
-            rc[it] = Cj[static_cast<size_t>(kk) * ldc + cc];
-        }
-    };
-    auto lstore = [&](int buf) {
-        const int J = lJ;
-#pragma unroll
-        for (int it = 0; it < NA; ++it) {
-            const ACoord ac = a_coord<KS>(tid + NTHR * it, I0, J);
-            As[buf][(ac.sub * 32 + ac.r) * AP + ac.c] = ra[it];
-        }
-#pragma unroll
-        for (int it = 0; it < NC; ++it) {
-            const int e = tid + NTHR * it;
-            const int kk = e / PAN, cc = e - kk * PAN;
-            Cs[buf][kk * CP + cc] = rc[it];
-        }
-    };
-    // Fragments of one k4 sub-step are loaded while the MFMAs of the previous one run
-    // (explicit double buffer; the scheduling barriers keep the compiler from hoisting
-    // every sub-step's loads, which would not fit in the register file).
-    auto frag = [&](int buf, int ks, T* a, T* b) {
-#pragma unroll
-        for (int i = 0; i < NI; ++i) a[i] = As[buf][(i * 16 + (lane & 15)) * AP + ks + (lane >> 4)];
-#pragma unroll
-        for (int jj = 0; jj < NBW; ++jj)
-            b[jj] = Cs[buf][(ks + (lane >> 4)) * CP + wn * NBW * 16 + jj * 16 + (lane & 15)];
-    };
-    auto mfma = [&](const T* a, const T* b) {
-#pragma unroll
-        for (int i = 0; i < NI; ++i)
-#pragma unroll
-            for (int jj = 0; jj < NBW; ++jj) {
-                if constexpr (F64)
-                    acc[i][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[jj], acc[i][jj], 0, 0, 0);
-                else
-                    acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[jj], acc[i][jj], 0, 0, 0);
-            }
-    };
-    auto compute = [&](int buf) {
-        T a0[NI], b0[NBW], a1[NI], b1[NBW];
-        frag(buf, 0, a0, b0);
-        frag(buf, 4, a1, b1);
-        __builtin_amdgcn_sched_barrier(0);
-        mfma(a0, b0);
-        __builtin_amdgcn_sched_barrier(0);
-        frag(buf, 8, a0, b0);
-        __builtin_amdgcn_sched_barrier(0);
-        mfma(a1, b1);
-        __builtin_amdgcn_sched_barrier(0);
-        frag(buf, 12, a1, b1);
-        __builtin_amdgcn_sched_barrier(0);
-        mfma(a0, b0);
-        __builtin_amdgcn_sched_barrier(0);
-        mfma(a1, b1);
-    };
-
-    if (nsteps > 0) {
-        gload(0);
-        lstore(0);
-        __syncthreads();
-        int buf = 0;
-        for (int st = 0; st < nsteps; ++st) {
-            const bool more = st + 1 < nsteps;
-            if (more) gload(st + 1);
-            compute(buf);
-            if (more) lstore(buf ^ 1);
-            __syncthreads();
-            buf ^= 1;
-        }
-    }
-    T* Xq = X + static_cast<size_t>(q) * xrows * ldc;
-#pragma unroll
-    for (int i = 0; i < NI; ++i)
-#pragma unroll
-        for (int j = 0; j < NBW; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int rr = F64 ? (4 * r + (lane >> 4)) : (4 * (lane >> 4) + r);
-                const int m = ROWS * rb + i * 16 + rr;
-                if (m < xrows)
-                    Xq[static_cast<size_t>(m) * ldc + col0 + wn * NBW * 16 + j * 16 + (lane & 15)] = acc[i][j][r];
-            }
-}
-
-// =====================================================================================
 // Half-transform, 3-term BF16 split: X = Bh Ch + Bh Cl + Bl Ch with FP32 accumulation.
 // B as hi/lo BF16 copies, C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one
 // column tile (32).  WM*256 threads, workgroup tile 64*WM rows x 64*NBW columns.
@@ -421,6 +273,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 // =====================================================================================
 template <int NBW, int WM>
 __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ht_bf3(const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl,
+                                                     const uint16_t* __restrict__ BhT, const uint16_t* __restrict__ BlT,
                                                      const int* __restrict__ tidx, int nT, int ntp, int q0, int nq,
                                                      int npanel, const uint16_t* __restrict__ CTh,
                                                      const uint16_t* __restrict__ CTl,
@@ -466,7 +319,6 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int sub = tid >> 7, f = tid & 127;
     uint4 rah, ral;
     uint4 c0h, c0l, c1h, c1l, c2h, c2l, c3h, c3l, c4h, c4l;
-    int at_flag = 0;
     // tile index of the next step to load, read one step ahead (see k_ht_mfma)
     int nJ = 0, noff = -1;
     auto fetch = [&](int st) {
@@ -476,9 +328,11 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     auto gload = [&](int st) {
         const int J = nJ;
         const int off = noff;
-        at_flag = (I0 + sub > J);
-        const uint16_t* th = off >= 0 ? Bh + qoff + off : ztile;
-        const uint16_t* tl = off >= 0 ? Bl + qoff + off : ztile;
+        // a block stored as (J, I) is read from the tile-transposed copies, so the vector
+        // below is always a row segment of this row tile's block
+        const bool tr = (I0 + sub > J);
+        const uint16_t* th = off >= 0 ? (tr ? BhT : Bh) + qoff + off : ztile;
+        const uint16_t* tl = off >= 0 ? (tr ? BlT : Bl) + qoff + off : ztile;
         // vector f covers stored-tile row (f>>2), cols 8*(f&3)..+7
         const int o = (f >> 2) * TILE + 8 * (f & 3);
         rah = *reinterpret_cast<const uint4*>(th + o);
@@ -506,20 +360,8 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         uint16_t* Ch = Al + A_SZ;
         uint16_t* Cl = Ch + C_SZ;
         const int r = f >> 2, c8 = 8 * (f & 3);
-        if (!at_flag) {
-            *reinterpret_cast<uint4*>(Ah + (sub * 32 + r) * KP + c8) = rah;
-            *reinterpret_cast<uint4*>(Al + (sub * 32 + r) * KP + c8) = ral;
-        } else {
-            // stored (J,I) tile: element [r][c8+e] is A[c8+e][r] of this row tile
-            const uint32_t hw[4] = {rah.x, rah.y, rah.z, rah.w};
-            const uint32_t lw[4] = {ral.x, ral.y, ral.z, ral.w};
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int sh = 16 * (e & 1);
-                Ah[(sub * 32 + c8 + e) * KP + r] = static_cast<uint16_t>(hw[e >> 1] >> sh);
-                Al[(sub * 32 + c8 + e) * KP + r] = static_cast<uint16_t>(lw[e >> 1] >> sh);
-            }
-        }
+        *reinterpret_cast<uint4*>(Ah + (sub * 32 + r) * KP + c8) = rah;
+        *reinterpret_cast<uint4*>(Al + (sub * 32 + r) * KP + c8) = ral;
         auto cstore = [&](int it, const uint4& h, const uint4& l) {
             const int v = tid + NTHR * it;
             if (it < NCV || v < NCV_R + NTHR * NCV) {
@@ -609,7 +451,8 @@ template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_mfma(const T* __restrict__ A, size_t strideA, const T* __restrict__ Bm,
                                                     size_t strideB, int ld, int kbeg, int kend, int nbatch,
                                                     int nsplit, const int2* __restrict__ tiles, int ntile,
-                                                    double* __restrict__ P, double alpha, int accumulate) {
+                                                    double* __restrict__ P, double alpha, int accumulate,
+                                                    int upper) {
     constexpr int KS = 16, AP = KS + 1;
     constexpr bool F64 = sizeof(T) == 8;
     typedef typename std::conditional<F64, d4_t, f4_t>::type acc_t;
@@ -621,6 +464,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (!xi.valid) return;
     const int tile = xi.item, split = xi.group;
     const int2 tb = tiles[tile];
+    // symmetric result: only the upper triangle of a diagonal tile is read, so the wave
+    // whose 64x64 block lies below it does the staging but no MFMAs
+    const bool idle = upper && tb.x == tb.y && wr > wc;
     const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
     const int b1 = static_cast<int>((static_cast<long long>(nbatch) * (split + 1)) / nsplit);
     const int nks = (kend - kbeg) / KS;
@@ -685,12 +531,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int s = 0; s < nsteps; ++s) {
             const bool more = s + 1 < nsteps;
             if (more) gload();
-            compute(buf);
+            if (!idle) compute(buf);
             if (more) lstore(buf ^ 1);
             __syncthreads();
             buf ^= 1;
         }
     }
+    if (idle) return;
     double* Pt = P + (static_cast<size_t>(split) * ntile + tile) * 16384;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -718,7 +565,7 @@ __device__ inline int kg_swz(int row, int chunk) { return row * 32 + ((chunk ^ (
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_bf3(
     const uint16_t* __restrict__ Ah_, const uint16_t* __restrict__ Al_, const uint16_t* __restrict__ Bh_,
     const uint16_t* __restrict__ Bl_, size_t stride, int ld, int kbeg, int kend, int nbatch, int nsplit,
-    const int2* __restrict__ tiles, int ntile, double* __restrict__ P, double alpha, int accumulate) {
+    const int2* __restrict__ tiles, int ntile, double* __restrict__ P, double alpha, int accumulate, int upper) {
     constexpr int KS = 32, SZ = 128 * 32;
     __shared__ uint16_t Ls[2][4 * SZ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -727,6 +574,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (!xi.valid) return;
     const int tile = xi.item, split = xi.group;
     const int2 tb = tiles[tile];
+    const bool idle = upper && tb.x == tb.y && wr > wc;  // below the diagonal of a diagonal tile
     const int m0 = tb.x * 128, n0 = tb.y * 128;
     const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
     const int b1 = static_cast<int>((static_cast<long long>(nbatch) * (split + 1)) / nsplit);
@@ -808,12 +656,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int s = 0; s < nsteps; ++s) {
             const bool more = s + 1 < nsteps;
             if (more) gload(s + 1);
-            compute(buf);
+            if (!idle) compute(buf);
             if (more) lstore(buf ^ 1);
             __syncthreads();
             buf ^= 1;
         }
     }
+    if (idle) return;
     double* Pt = P + (static_cast<size_t>(split) * ntile + tile) * 16384;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -840,6 +689,21 @@ __global__ void k_to_bf_split(const double* __restrict__ src, uint16_t* __restri
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
         const float x = static_cast<float>(src[i]);
+        const uint16_t h = f2bf(x);
+        hi[i] = h;
+        lo[i] = f2bf(x - bf2f(h));
+    }
+}
+
+// Same split, each 32x32 tile transposed: out[t][r][c] = split(src[t][c][r]).  The half-transform
+// reads these for blocks it needs in the transposed orientation, so every LDS store is a
+// 16-byte row segment.
+__global__ void k_to_bf_split_t(const double* __restrict__ src, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t e = i % TILE2, base = i - e;
+        const float x = static_cast<float>(src[base + (e % TILE) * TILE + e / TILE]);
         const uint16_t h = f2bf(x);
         hi[i] = h;
         lo[i] = f2bf(x - bf2f(h));
@@ -933,20 +797,6 @@ static void ht_mfma_wm(const TileGeom& g, const T* B, int q0, int nq, const T* C
 }
 
 int g_ht_wm = 0;       // 0: default shape; 1 or 2 force the row-wave count (tuning sweeps)
-int g_ht_variant = 0;  // 0: two waves per SIMD (k_ht_mfma); 1: wide waves (k_ht_wide)
-
-// FP64 accumulators of 2*RT*NBW 16x16 tiles must fit the 256 accumulation registers
-// (RT = 3 at NBW = 5); FP32 takes four row tiles.
-template <typename T, int NBW>
-static void ht_wide_nbw(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int npanel, T* X,
-                        int xrows, const void* ztile, hipStream_t s) {
-    constexpr int RT = sizeof(T) == 8 ? (NBW == 5 ? 3 : 4) : 4;
-    const int nrb = (g.nT + RT - 1) / RT;
-    require(static_cast<size_t>(nrb) * RT * 32 >= static_cast<size_t>(g.nbfp), "ht_wide rows");
-    dim3 grid(xcd_grid(nq, nrb * npanel));
-    hipLaunchKernelGGL((k_ht_wide<T, NBW, RT>), grid, dim3(256), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, C,
-                       ldc, X, xrows, static_cast<const T*>(ztile));
-}
 
 template <typename T>
 static void ht_mfma(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int nbw, int npanel, T* X,
@@ -954,14 +804,6 @@ static void ht_mfma(const TileGeom& g, const T* B, int q0, int nq, const T* C, i
     require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 &&
                 xrows == (g.nT + 3) / 4 * 128 && g.nT <= kMaxColTiles && zt != nullptr,
             "ht_mfma shape");
-    if (g_ht_variant == 1 && nbw >= 3) {
-        switch (nbw) {
-            case 3: ht_wide_nbw<T, 3>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
-            case 4: ht_wide_nbw<T, 4>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
-            default: ht_wide_nbw<T, 5>(g, B, q0, nq, C, ldc, npanel, X, xrows, zt, s); break;
-        }
-        return;
-    }
     const int wm = g_ht_wm ? g_ht_wm : 2;
     if (wm == 1)
         ht_mfma_wm<T, 1>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
@@ -978,10 +820,10 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
     ht_mfma<float>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
 }
 void set_ht_rowwaves(int wm) { g_ht_wm = (wm == 1 || wm == 2) ? wm : 0; }
-void set_ht_variant(int v) { g_ht_variant = (v == 1) ? 1 : 0; }
 
 template <int WM>
-static void ht_bf3_wm(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
+static void ht_bf3_wm(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, const uint16_t* BhT,
+                      const uint16_t* BlT, int q0, int nq, const uint16_t* CTh,
                       const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
                       const void* zt, hipStream_t s) {
     dim3 grid(xcd_grid(nq, xrows / (64 * WM) * npanel));
@@ -989,7 +831,7 @@ static void ht_bf3_wm(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl,
     switch (nbw) {
 #define P4A_BF3(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_ht_bf3<n, WM>), grid, dim3(WM * 256), 0, s, Bh, Bl, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, \
+        hipLaunchKernelGGL((k_ht_bf3<n, WM>), grid, dim3(WM * 256), 0, s, Bh, Bl, BhT, BlT, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, \
                            CTh, CTl, g.nbfp, ldc, Xh, Xl, xrows, z);                                                \
         break;
         P4A_BF3(1) P4A_BF3(2) P4A_BF3(3) P4A_BF3(4) P4A_BF3(5)
@@ -997,7 +839,8 @@ static void ht_bf3_wm(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl,
     }
 }
 
-void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
+void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, const uint16_t* BhT,
+                   const uint16_t* BlT, int q0, int nq, const uint16_t* CTh,
                    const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
                    const void* zt, hipStream_t s) {
     require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 && g.nbfp % 32 == 0 &&
@@ -1005,9 +848,9 @@ void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, in
             "ht_bf3 shape");
     const int wm = g_ht_wm ? g_ht_wm : 2;
     if (wm == 1)
-        ht_bf3_wm<1>(g, Bh, Bl, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+        ht_bf3_wm<1>(g, Bh, Bl, BhT, BlT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
     else
-        ht_bf3_wm<2>(g, Bh, Bl, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+        ht_bf3_wm<2>(g, Bh, Bl, BhT, BlT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
 }
 
 static void kgemm_check(int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, int ntile, int ks) {
@@ -1018,24 +861,24 @@ static void kgemm_check(int xrows, int ld, int kbeg, int kend, int nbatch, int n
 
 void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrows, int ld, int kbeg, int kend,
                        int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
-                       hipStream_t s) {
+                       int upper, hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
     hipLaunchKernelGGL((k_kgemm_mfma<double>), dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, A, stride, B, stride, ld,
-                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate);
+                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper);
 }
 void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int ld, int kbeg, int kend, int nbatch,
-                       int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
+                       int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate, int upper,
                        hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
     hipLaunchKernelGGL((k_kgemm_mfma<float>), dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, A, stride, B, stride, ld,
-                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate);
+                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper);
 }
 void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh, const uint16_t* Bl, size_t stride,
                       int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles, int ntile,
-                      double* P, double alpha, int accumulate, hipStream_t s) {
+                      double* P, double alpha, int accumulate, int upper, hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 32);
     hipLaunchKernelGGL(k_kgemm_bf3, dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, Ah, Al, Bh, Bl, stride, ld, kbeg,
-                       kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate);
+                       kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper);
 }
 
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s) {
@@ -1043,6 +886,10 @@ void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s) {
 }
 void launch_to_bf_split(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_bf_split, dim3(nblocks(n, 256)), dim3(256), 0, s, src, hi, lo, n);
+}
+void launch_to_bf_split_t(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s) {
+    require(n % TILE2 == 0, "to_bf_split_t");
+    hipLaunchKernelGGL(k_to_bf_split_t, dim3(nblocks(n, 256)), dim3(256), 0, s, src, hi, lo, n);
 }
 void launch_pack_cmap_f64(const double* src, int ldsrc, int nbf, const int* cmap, double* dst, int nbfp, int ldc,
                           hipStream_t s) {

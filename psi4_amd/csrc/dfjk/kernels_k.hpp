@@ -21,10 +21,9 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
                     float* X, int xrows, const void* zt, hipStream_t s);
 // Row waves per half-transform workgroup: 1 (256 threads) or 2 (512 threads); 0 = default.
 void set_ht_rowwaves(int wm);
-// Half-transform shape for FP64/FP32 panels of 3-5 column blocks: 0 two waves per SIMD, 1 wide waves.
-void set_ht_variant(int v);
 // 3-term BF16: B hi/lo [naux][ntp][32][32], C^T hi/lo [ldc][nbfp], X hi/lo [nq][xrows][ldc].
-void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, int q0, int nq, const uint16_t* CTh,
+void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, const uint16_t* BhT,
+                   const uint16_t* BlT, int q0, int nq, const uint16_t* CTh,
                    const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
                    const void* zt, hipStream_t s);
 
@@ -32,17 +31,19 @@ void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, in
 // A_b = A + b*stride, rows of length ld, xrows rows (a multiple of 128).
 void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrows, int ld, int kbeg, int kend,
                        int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
-                       hipStream_t s);
+                       int upper, hipStream_t s);
 void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int ld, int kbeg, int kend, int nbatch,
                        int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
-                       hipStream_t s);
+                       int upper, hipStream_t s);
 void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh, const uint16_t* Bl, size_t stride,
                       int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles, int ntile,
-                      double* P, double alpha, int accumulate, hipStream_t s);
+                      double* P, double alpha, int accumulate, int upper, hipStream_t s);
 
 // Precision copies and operand packing (cmap: destination column -> source column or -1).
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
 void launch_to_bf_split(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s);
+// hi/lo split with every 32x32 tile transposed (the blocks the half-transform reads as (J,I)).
+void launch_to_bf_split_t(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s);
 void launch_pack_cmap_f64(const double* src, int ldsrc, int nbf, const int* cmap, double* dst, int nbfp, int ldc,
                           hipStream_t s);
 void launch_pack_cmap_f32(const double* src, int ldsrc, int nbf, const int* cmap, float* dst, int nbfp, int ldc,

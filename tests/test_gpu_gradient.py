@@ -55,7 +55,8 @@ def test_h2o4_gradient_device_vs_oracle_adjoint(gpu_available):
     assert err < 1e-9
     assert abs(terms["E_2"] - to["E_2"]) < 1e-11 * abs(to["E_2"])
     assert np.abs(g.sum(axis=0)).max() < 1e-9
-    assert np.abs(np.cross(mol.xyz, g).sum(axis=0)).max() < 1e-8
+    # torque-free up to the SCF convergence (d_convergence 1e-9) and the 1e-12 Schwarz screening
+    assert np.abs(np.cross(mol.xyz, g).sum(axis=0)).max() < 1e-7
 
 
 def test_scf6_uhf_energy_and_gradient_on_gpu(gpu_available):
