@@ -255,9 +255,14 @@ void build_tiles(dfjk_handle* h) {
     h->otile_of_all = of_all;
 }
 
+int g_kgemm_wgs = 2048;  // workgroups the Q split of the contraction aims at (tuning: PSI4AMD_KGEMM_WGS)
+size_t g_x_budget = size_t(4) << 30;  // bytes of X workspace without a memory setting (PSI4AMD_X_BUDGET_GB)
+
 void upload_tiles(dfjk_handle* h) {
     hcheck(hipMemset(h->dZero.get<double>(TILE2), 0, TILE2 * sizeof(double)), "zero tile");
     if (const char* e = std::getenv("PSI4AMD_HT_ROWWAVES")) set_ht_rowwaves(std::atoi(e));
+    if (const char* e = std::getenv("PSI4AMD_KGEMM_WGS")) g_kgemm_wgs = std::max(256, std::atoi(e));
+    if (const char* e = std::getenv("PSI4AMD_X_BUDGET_GB")) g_x_budget = static_cast<size_t>(std::max(1, std::atoi(e))) << 30;
     const auto& sym = h->otiles_sym;
     const auto& all = h->otiles_all;
     const auto& of_sym = h->otile_of_sym;
@@ -376,7 +381,7 @@ void initialize_basis(dfjk_handle* h) {
 // Q-splits of the contraction: about 2048 workgroups, a multiple of the 8 XCDs when
 // the batch allows (the contraction maps each split to one XCD).
 int choose_nsplit(int ntile, int nbatch) {
-    const int target = 2048;
+    const int target = g_kgemm_wgs;
     int ns = std::max(1, (target + ntile - 1) / ntile);
     ns = (ns + 7) / 8 * 8;
     return std::min(ns, std::max(1, nbatch));
@@ -552,7 +557,7 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         const int prec = h->precision;
         const size_t xq = static_cast<size_t>(xrows) * ldc;  // elements per Q
         const size_t esz = prec == DFJK_PREC_FP64 ? 8 : 4;    // FP64, FP32, or BF16 hi+lo
-        size_t budget = size_t(4) << 30;                      // bytes of X workspace
+        size_t budget = g_x_budget;                           // bytes of X workspace
         if (h->memory_doubles) {
             const size_t resident = (b_doubles(h) + static_cast<size_t>(h->naux) * h->naux) * sizeof(double);
             const size_t mem = h->memory_doubles * sizeof(double);

@@ -75,6 +75,8 @@ inline unsigned xcd_grid(int ngroups, int items) {
 // in B_Q (or -1), built once in LDS at kernel start (parallel presence test and
 // an inclusive scan), so the K loop does no dependent tile-table lookups.
 constexpr int kMaxColTiles = 256;  // nbf <= 8192
+// k-terms an FP32 MFMA accumulator of the K contraction holds before it is folded into FP64
+constexpr int kFold32 = 512;
 
 template <int RT>
 struct StepList {
@@ -216,6 +218,38 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             Cs[buf][kk * CP + cc] = rc[it];
         }
     };
+    // fragments of a whole step (KS/4 sub-steps), read before the next step's loads are
+    // issued so their LDS latency overlaps the index lookups and address arithmetic
+    constexpr int NSUB = KS / 4;
+    // sub-steps whose fragments are read before the loads (the FP64 5-block shape has no
+    // register room for them next to the staging registers and reads them after)
+    constexpr int NPRE = (F64 && NBW == 5) ? 0 : NSUB;
+    T fa[NSUB][4], fb[NSUB][NBW];
+    auto frags = [&](int buf, int u0, int u1) {
+#pragma unroll
+        for (int u = u0; u < u1; ++u) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                fa[u][i] = As[buf][(wm * 64 + i * 16 + (lane & 15)) * AP + 4 * u + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < NBW; ++j)
+                fb[u][j] = Cs[buf][(4 * u + (lane >> 4)) * CP + wn * NBW * 16 + j * 16 + (lane & 15)];
+        }
+    };
+    auto mma = [&]() {
+#pragma unroll
+        for (int u = 0; u < NSUB; ++u)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < NBW; ++j) {
+                    if constexpr (F64)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][i], fb[u][j], acc[i][j], 0, 0, 0);
+                    else
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[u][i], fb[u][j], acc[i][j], 0, 0, 0);
+                }
+    };
+    // per-sub-step reads and MFMAs, for the shape that reads its fragments after the loads
     auto compute = [&](int buf) {
 #pragma unroll
         for (int ks = 0; ks < KS; ks += 4) {
@@ -244,8 +278,15 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         int buf = 0;
         for (int st = 0; st < nsteps; ++st) {
             const bool more = st + 1 < nsteps;
-            if (more) gload(st + 1);
-            compute(buf);
+            if constexpr (NPRE > 0) {
+                frags(buf, 0, NPRE);
+                if (more) gload(st + 1);
+                if (NPRE < NSUB) frags(buf, NPRE, NSUB);
+                mma();
+            } else {
+                if (more) gload(st + 1);
+                compute(buf);
+            }
             if (more) lstore(buf ^ 1, st + 1);
             __syncthreads();
             buf ^= 1;
@@ -446,6 +487,9 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 // K contraction, FP64 / FP32 (16x16x4 MFMA): P[split][tile] (+)= sum_b A_b S B_b^T over
 // 128x128 output tiles, 4 waves (2x2) of 64x64, K-step 16, double-buffered LDS.
 // S = diag(sign) if sign != nullptr.  Accumulators are stored to FP64 partials.
+// FP32: the MFMA accumulators hold at most kFold32 k-terms before they are added into FP64
+// totals, so the rounding error of a sum does not grow with the number of auxiliary
+// functions a split covers.
 // =====================================================================================
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_mfma(const T* __restrict__ A, size_t strideA, const T* __restrict__ Bm,
@@ -479,6 +523,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     acc_t acc[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = acc_t{0, 0, 0, 0};
+    // FP64 totals of the folded FP32 accumulators (unused for FP64)
+    constexpr int FOLD = F64 ? 0 : kFold32 / KS;
+    d4_t tot[F64 ? 1 : 16];
+#pragma unroll
+    for (int i = 0; i < (F64 ? 1 : 16); ++i) tot[i] = d4_t{0, 0, 0, 0};
+    auto fold = [&]() {
+        if constexpr (!F64) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tot[i][r] += static_cast<double>(acc[i][r]);
+                acc[i] = acc_t{0, 0, 0, 0};
+            }
+        }
+    };
 
     T ra[8], rb[8];
     int b = b0, ks0 = 0;
@@ -527,17 +586,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         gload();
         lstore(0);
         __syncthreads();
-        int buf = 0;
+        int buf = 0, nf = 0;
         for (int s = 0; s < nsteps; ++s) {
             const bool more = s + 1 < nsteps;
             if (more) gload();
-            if (!idle) compute(buf);
+            if (!idle) {
+                compute(buf);
+                if constexpr (!F64) {
+                    if (++nf == FOLD) {
+                        fold();
+                        nf = 0;
+                    }
+                }
+            }
             if (more) lstore(buf ^ 1);
             __syncthreads();
             buf ^= 1;
         }
     }
     if (idle) return;
+    fold();
     double* Pt = P + (static_cast<size_t>(split) * ntile + tile) * 16384;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -549,7 +617,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const int row = wr * 64 + i * 16 + rr;
                 const int col = wc * 64 + j * 16 + (lane & 15);
                 double* o = Pt + row * 128 + col;
-                *o = alpha * static_cast<double>(acc[i * 4 + j][r]) + (accumulate ? *o : 0.0);
+                double v;
+                if constexpr (F64) v = acc[i * 4 + j][r];
+                else v = tot[i * 4 + j][r];
+                *o = alpha * v + (accumulate ? *o : 0.0);
             }
 }
 
@@ -559,6 +630,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // 64 bytes (32 k) with the four 16-byte chunks of row r stored at chunk c ^ ((r>>1)&3):
 // the fragment reads (16 rows x one chunk per 16 lanes) and the staging writes are
 // bank-conflict free, and the 64 KB footprint leaves room for two workgroups per CU.
+// The MFMA accumulators are folded into a second set of FP32 totals every kFold32 k-terms
+// (FP64 totals would not fit beside the fragments in 256 registers): a sum of N terms then
+// rounds like two chains of about sqrt(N) additions instead of one chain of N.
 // =====================================================================================
 __device__ inline int kg_swz(int row, int chunk) { return row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3); }
 
@@ -581,9 +655,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int nks = (kend - kbeg) / KS;
     const int nsteps = (b1 - b0) * nks;
 
-    f4_t acc[16];
+    f4_t acc[16], tot[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = f4_t{0, 0, 0, 0};
+    for (int i = 0; i < 16; ++i) {
+        acc[i] = f4_t{0, 0, 0, 0};
+        tot[i] = f4_t{0, 0, 0, 0};
+    }
+    auto fold = [&]() {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            tot[i] += acc[i];
+            acc[i] = f4_t{0, 0, 0, 0};
+        }
+    };
 
     // per step: 4 arrays x 128 rows x 32 k = 4 x 512 vectors of 8; 256 threads -> 2 vectors per array
     // thread's two 8-element vectors per array: rows (tid>>2) and (tid>>2)+64, chunk tid&3
@@ -652,17 +736,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         gload(0);
         lstore(0);
         __syncthreads();
-        int buf = 0;
+        int buf = 0, nf = 0;
         for (int s = 0; s < nsteps; ++s) {
             const bool more = s + 1 < nsteps;
             if (more) gload(s + 1);
-            if (!idle) compute(buf);
+            if (!idle) {
+                compute(buf);
+                if (++nf == kFold32 / KS) {
+                    fold();
+                    nf = 0;
+                }
+            }
             if (more) lstore(buf ^ 1);
             __syncthreads();
             buf ^= 1;
         }
     }
     if (idle) return;
+    fold();
     double* Pt = P + (static_cast<size_t>(split) * ntile + tile) * 16384;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -673,7 +764,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const int row = wr * 64 + i * 16 + 4 * (lane >> 4) + rr;
                 const int col = wc * 64 + j * 16 + (lane & 15);
                 double* o = Pt + row * 128 + col;
-                *o = alpha * static_cast<double>(acc[i * 4 + j][rr]) + (accumulate ? *o : 0.0);
+                *o = alpha * static_cast<double>(tot[i * 4 + j][rr]) + (accumulate ? *o : 0.0);
             }
 }
 
