@@ -4,12 +4,18 @@
 Metric: J+K builds/s at nbf=1500, naux=6000, nocc=300 (mixed vs FP64), |dE| vs FP64.
 Workload: the seeded synthetic tensor-level case -- B_Qmn = s_Q exp(-|m-n|/64)
 (G_Qmn+G_Qnm)/2 generated on the device, C_occ = first 300 columns of the QR
-factor of a seeded N(0,1) 1500x1500 matrix.  One step = one symmetric J+K build
-(D = C C^T, J, K) with C already resident in HBM and J, K left in HBM.
+factor of a seeded N(0,1) 1500x1500 matrix.
 
-Printed: one JSON line (rank 0) with the headline precision's builds/s as
-``value``, the FP64 builds/s beside it, |dE| = |E_mixed - E_fp64| with
-E = tr(D(2J-K)), the roofline of the dominant kernel (HIP events on the
+A mixed-precision step is the Fock build the way SCF uses it: J(D_new), K(D_new)
+from FP64 J, K of the previous density plus J(dD), K(dD) computed in the mixed
+precision, with dD = C_new C_new^T - C_old C_old^T passed in the cross form
+C_left = [A, V], C_right = [V, A] (A = (C_new + C_old)/2, V = C_new - C_old).
+The FP64 step is one plain symmetric J+K build of D_new.  C is resident in HBM
+and J, K are left in HBM.  Full builds in FP32 and 3xBF16 are timed beside them.
+
+Printed: one JSON line (rank 0) with the headline precision's step rate as
+``value``, the FP64 builds/s beside it, |dE| = |E_step - E_fp64| with
+E = tr(D(2J-K)), the roofline of the step's dominant kernel (HIP events on the
 library's stream over the timed region), and the CPU oracle timed on a
 bounded Q-slice of the same workload (J and K are additive over Q).
 """
@@ -37,7 +43,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16x3", help="headline (mixed) precision: fp32|bf16x3|fp64")
-    ap.add_argument("--no-incremental", action="store_true", help="skip the incremental (dD) build measurement")
+    ap.add_argument("--no-incremental", action="store_true",
+                    help="time full builds only (the headline is then the full mixed build)")
+    ap.add_argument("--incremental-only", action="store_true",
+                    help="time only the incremental mixed step (no full builds, no |dE|): for counter runs")
     ap.add_argument("--cpu-sample", type=int, default=1800, help="aux functions in the CPU oracle slice (0: skip)")
     ap.add_argument("--nbf", type=int, default=None)
     ap.add_argument("--naux", type=int, default=None)
@@ -96,10 +105,12 @@ def scf_workload(args):
     out = {"workload": f"RHF (H2O)_{nx * ny * nz} cc-pVDZ/cc-pVDZ-jkfit, seeded cluster",
            "nbf": nbf, "naux": naux, "nocc": int(mixed.nocc), "setup_s": round(t_setup, 2),
            "plain_fp64": {"E": rp.energy, "iterations": rp.iterations, "wall_s": round(t_plain, 3),
-                          "jk_s": round(rp.timings["jk_s"], 3)},
+                          "jk_s": round(rp.timings["jk_s"], 3),
+                          "jk_ms_per_iteration": [round(h["jk_s"] * 1e3, 2) for h in rp.history]},
            "mixed": {"precision": prec, "E": rm.energy, "iterations": rm.iterations,
                      "mixed_builds": kinds.count(prec), "wall_s": round(t_mixed, 3),
-                     "jk_s": round(rm.timings["jk_s"], 3)},
+                     "jk_s": round(rm.timings["jk_s"], 3),
+                     "jk_ms_per_iteration": [[h["build"], round(h["jk_s"] * 1e3, 2)] for h in rm.history]},
            "dE_mixed_vs_plain": abs(rm.energy - rp.energy),
            "adjoint": {"wall_s": round(t_adj, 3), "E_2": E2, "rel_<B,dE/dB>/2E-1": euler_B,
                        "rel_<C,dE/dC>/4E-1": euler_C}}
@@ -188,6 +199,9 @@ def main():
 
     results = {}
     order = [headline] + [p for p in ("fp32", "fp64") if p != headline and not (p == "fp64" and args.skip_fp64)]
+    if args.incremental_only:
+        order = []
+        D.copy_(C @ C.T)
     for p in order:
         jk.set_precision(prec_id[p])
         jk.set_profiling(True)
@@ -216,7 +230,8 @@ def main():
     # and the engine contracts it as a symmetric K.  The same build in FP64 checks the
     # identity K(D_old) + K(dD) = K(D_new) at this size.
     inc = None
-    if not args.no_incremental:
+    tinc = None
+    if not args.no_incremental and headline != "fp64":
         rng = np.random.default_rng(SYNTHETIC["seed_C"] + 1)
         Cold_h = np.linalg.qr(C_h + 1e-4 * rng.standard_normal(C_h.shape))[0]
         u, _, vt = np.linalg.svd(Cold_h.T @ C_h)
@@ -245,14 +260,20 @@ def main():
             E_inc["fp64"] = float(torch.sum(D * (2.0 * Ji - Ki)).item())
         jk.set_precision(prec_id[headline])
         log(f"incremental {headline}: {args.warmup} warmup + {args.steps} timed dD builds (ncol={2 * nocc})")
+        jk.set_profiling(True)
         for _ in range(args.warmup):
             inc_build()
         torch.cuda.synchronize()
+        tinc = {"J": 0.0, "HT": 0.0, "KGEMM": 0.0, "other": 0.0, "n_ht": 0, "n_kgemm": 0}
         t0 = time.perf_counter()
         for _ in range(args.steps):
             inc_build()
+            t = jk.timings()
+            for k in tinc:
+                tinc[k] += t[k]
         torch.cuda.synchronize()
         dt_inc = (time.perf_counter() - t0) / args.steps
+        jk.set_profiling(False)
         E_inc[headline] = float(torch.sum(D * (2.0 * Ji - Ki)).item())
         dD = float(np.abs(C_h @ C_h.T - Cold_h @ Cold_h.T).max())
         ef = results["fp64"]["E"] if "fp64" in results else None
@@ -262,14 +283,28 @@ def main():
                "max_abs_dD": dD, "max_abs_V": float(np.abs(V_h).max()),
                "dE_vs_fp64": (abs(E_inc[headline] - ef) if ef is not None else None),
                "dE_fp64_incremental_vs_fp64_full": (abs(E_inc["fp64"] - ef) if ef is not None else None)}
+        inc["kernel_ms_per_build"] = {k: round(v / args.steps, 4) for k, v in tinc.items() if not k.startswith("n_")}
         log(f"incremental {headline}: {dt_inc * 1e3:.2f} ms/build, |dD|max={dD:.2e}, dE={inc['dE_vs_fp64']}, "
-            f"FP64 incremental dE={inc['dE_fp64_incremental_vs_fp64_full']}")
+            f"FP64 incremental dE={inc['dE_fp64_incremental_vs_fp64_full']}, kernels(ms)=J "
+            f"{tinc['J'] / args.steps:.2f} HT {tinc['HT'] / args.steps:.2f} KGEMM {tinc['KGEMM'] / args.steps:.2f}")
         jk.set_precision(prec_id[headline])
 
-    # roofline of the dominant kernel class of the headline build
-    hr = results[headline]
-    tt = hr["timings"]
-    fl_j, fl_ht, fl_kc = jk_flops(nbf, naux, nocc, symmetric=True)
+    # the headline step: the incremental mixed build when it ran, else the full build
+    step_kind = "incremental" if inc is not None else "full"
+    if step_kind == "incremental":
+        s = dt_inc
+        tt = tinc
+        dE_step = inc["dE_vs_fp64"]
+        ncol_ht = 2 * nocc      # C_left = [A, V]; the contraction makes two passes of nocc columns
+    else:
+        hr = results[headline]
+        s = hr["s_per_build"]
+        tt = hr["timings"]
+        dE_step = abs(hr["E"] - results["fp64"]["E"]) if "fp64" in results else None
+        ncol_ht = nocc
+
+    # roofline of the dominant kernel class of the headline step
+    fl_j, fl_ht, fl_kc = jk_flops(nbf, naux, ncol_ht, symmetric=True)
     dom = "HT" if tt["HT"] >= tt["KGEMM"] else "KGEMM"
     nlaunch = tt["n_ht"] if dom == "HT" else tt["n_kgemm"]
     flops_per_launch = (fl_ht if dom == "HT" else fl_kc) * args.steps / max(1, nlaunch)
@@ -283,9 +318,10 @@ def main():
                            + " (K = sum_Q X_Q X_Q^T)"),
                 "launches_per_build": nlaunch / args.steps, "ms_per_launch": round(ms_per_launch, 4),
                 "algorithmic_flops_per_launch": flops_per_launch}
-    # J path: bandwidth view (two streaming passes over packed B)
+    # J path: bandwidth view (two streaming passes over packed B: FP64, or the FP32 copy /
+    # BF16 hi+lo pair in the mixed precisions)
     ntp = jk.info()["ntile_pairs"]
-    j_bytes = 2 * naux * ntp * 1024 * 8
+    j_bytes = 2 * naux * ntp * 1024 * (8 if headline == "fp64" else 4)
     j_ms = tt["J"] / args.steps
     j_roof = {"bound": "hbm", "achieved_GBs": round(j_bytes / (j_ms * 1e-3) / 1e9, 1), "peak_GBs": HBM_PEAK_GBS,
               "bytes_per_build": j_bytes, "ms_per_build": round(j_ms, 4)}
@@ -296,7 +332,7 @@ def main():
                          f"traffic_synthetic_nbf{nbf}_naux{naux}_nocc{nocc}.json")
     kname = roofline["kernel"].split(" ")[0]
     if os.path.exists(tfile):
-        rec = json.load(open(tfile)).get("kernels", {}).get(kname)
+        rec = json.load(open(tfile)).get("steps", {}).get(step_kind, {}).get(kname)
         if rec:
             # bytes read + written by all dispatches of one build, over the aux functions they cover
             per_q = (rec["read_bytes"] + rec["write_bytes"]) / rec["q_covered"]
@@ -346,7 +382,6 @@ def main():
                "slice_maxabs_dJ": dj, "slice_maxabs_dK": dk}
         log(f"CPU slice {tcpu:.2f}s -> {cpu['value']} builds/s; GPU-vs-CPU slice |dJ|={dj:.2e} |dK|={dk:.2e}")
 
-    s = hr["s_per_build"]
     out = {
         "metric": "J+K builds/s at nbf=1500,naux=6000,nocc=300 (mixed vs FP64); |dE| vs FP64",
         "value": round(1.0 / s, 4),
@@ -361,11 +396,15 @@ def main():
         "dtype": {"fp64": "f64", "fp32": "f32 (K), f64 (J, accumulation)",
                   "bf16x3": "bf16x3 split, f32 accumulate (K); f64 (J, accumulation)"}[headline],
         "data": "synthetic (device Philox4x32-10 B, seeded QR C_occ)",
-        "config": {"workload": f"synthetic tensor-level DF-JK nbf={nbf} naux={naux} nocc={nocc}", "nbf": nbf,
-                   "naux": naux, "nocc": nocc, "precision": headline, "parallelism": "single GPU"},
+        "config": {"workload": (f"synthetic tensor-level DF-JK nbf={nbf} naux={naux} nocc={nocc}, "
+                                + ("incremental Fock build: FP64 J/K of D_old + J/K(dD) in " + headline
+                                   + " (cross form, 2*nocc columns)" if step_kind == "incremental"
+                                   else f"full {headline} J+K build")),
+                   "step": step_kind, "nbf": nbf, "naux": naux, "nocc": nocc, "precision": headline,
+                   "parallelism": "single GPU"},
         "fp64_builds_per_s": round(1.0 / results["fp64"]["s_per_build"], 4) if "fp64" in results else None,
         "speedup_vs_fp64": (round(results["fp64"]["s_per_build"] / s, 3) if "fp64" in results else None),
-        "dE_vs_fp64": (abs(results[headline]["E"] - results["fp64"]["E"]) if "fp64" in results else None),
+        "dE_vs_fp64": dE_step,
         "E_fp64": results.get("fp64", {}).get("E"),
         "by_precision": {p: {"builds_per_s": round(1.0 / r["s_per_build"], 4), "E": r["E"],
                              "dE_vs_fp64": (abs(r["E"] - results["fp64"]["E"]) if "fp64" in results else None),

@@ -193,14 +193,15 @@ class RHF:
                 K = K + dK
                 kind = self.mixed
                 since_full += 1
-            t_jk += time.perf_counter() - tj
+            dt_jk = time.perf_counter() - tj
+            t_jk += dt_jk
             if self.jk_callback is not None:
                 self.jk_callback(it, Cocc, J, K, kind)
             F = H + 2.0 * J - K
             E = enuc + 2.0 * np.vdot(D, H) + np.vdot(D, 2.0 * J - K)
             grad = X.T @ (F @ D @ S - S @ D @ F) @ X
             d_rms = float(np.sqrt(np.mean(grad ** 2)))
-            history.append(dict(iteration=it, energy=E, d_rms=d_rms, build=kind))
+            history.append(dict(iteration=it, energy=E, d_rms=d_rms, build=kind, jk_s=dt_jk))
             converged = abs(E - E_old) < self.e_conv and d_rms < self.d_conv
             if converged and kind == "fp64":
                 self.Ca, self.Fa, self.eps = C, F, eps
