@@ -135,7 +135,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     constexpr int RT = 2 * WM;            // row tiles per workgroup
     constexpr int ROWS = 64 * WM;
     constexpr int PAN = 64 * NBW;
-    constexpr int KS = F64 ? 8 : 16;
+    constexpr int KS = F64 ? (WM == 1 ? 4 : 8) : 16;
     constexpr int AP = KS + 1;
     constexpr int CP = PAN + 4;
     constexpr int NA = ROWS * KS / NTHR;  // A elements per thread per step
@@ -358,17 +358,18 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 
     // A: per row tile 32x32 bf16 = 128 vectors of 8; RT row tiles -> one vector per thread per array.
     const int sub = tid >> 7, f = tid & 127;
-    uint4 rah, ral;
-    uint4 c0h, c0l, c1h, c1l, c2h, c2l, c3h, c3l, c4h, c4l;
-    // tile index of the next step to load, read one step ahead (see k_ht_mfma)
-    int nJ = 0, noff = -1;
-    auto fetch = [&](int st) {
-        nJ = sl.J[st];
-        noff = sl.off[nJ * RT + sub];
-    };
-    auto gload = [&](int st) {
-        const int J = nJ;
-        const int off = noff;
+    // Staging registers.  The A vectors (B_Q blocks, streamed from HBM) are loaded two steps
+    // before their LDS store, in two sets; the C vectors (a panel every workgroup re-reads,
+    // mostly L2 hits) one step before.  Within a step the C loads are issued first, so the
+    // store's wait for them leaves the younger A loads in flight.
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    v4u a0h, a0l, a1h, a1l;
+    v4u c0h, c0l, c1h, c1l, c2h, c2l, c3h, c3l, c4h, c4l;
+    // column tile of step st; steps past the end re-read the last one (their data is never used)
+    auto jof = [&](int st) { return sl.J[min(st, nsteps - 1)]; };
+    auto gloadA = [&](int st, v4u& ah, v4u& al) {
+        const int J = jof(st);
+        const int off = sl.off[J * RT + sub];
         // a block stored as (J, I) is read from the tile-transposed copies, so the vector
         // below is always a row segment of this row tile's block
         const bool tr = (I0 + sub > J);
@@ -376,40 +377,46 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const uint16_t* tl = off >= 0 ? (tr ? BlT : Bl) + qoff + off : ztile;
         // vector f covers stored-tile row (f>>2), cols 8*(f&3)..+7
         const int o = (f >> 2) * TILE + 8 * (f & 3);
-        rah = *reinterpret_cast<const uint4*>(th + o);
-        ral = *reinterpret_cast<const uint4*>(tl + o);
-        const size_t kb = static_cast<size_t>(J) * TILE;
-        auto cload = [&](int it, uint4& h, uint4& l) {
-            const int v = tid + NTHR * it;
-            if (it < NCV || v < NCV_R + NTHR * NCV) {
-                const int col = v >> 2, k8 = 8 * (v & 3);
-                const size_t co = static_cast<size_t>(col0 + col) * nbfp + kb + k8;
-                h = *reinterpret_cast<const uint4*>(CTh + co);
-                l = *reinterpret_cast<const uint4*>(CTl + co);
-            }
+        ah = *reinterpret_cast<const v4u*>(th + o);
+        al = *reinterpret_cast<const v4u*>(tl + o);
+    };
+    auto gloadC = [&](int st) {
+        const size_t kb = static_cast<size_t>(jof(st)) * TILE;
+        // every lane loads (idle lanes of the partial round re-read the last vector and do
+        // not store it), so every step issues the same loads
+        auto cload = [&](int it, v4u& h, v4u& l) {
+            const int v = min(tid + NTHR * it, PAN * 4 - 1);
+            const int col = v >> 2, k8 = 8 * (v & 3);
+            const size_t co = static_cast<size_t>(col0 + col) * nbfp + kb + k8;
+            h = *reinterpret_cast<const v4u*>(CTh + co);
+            l = *reinterpret_cast<const v4u*>(CTl + co);
         };
         cload(0, c0h, c0l);
         if (NCV >= 1) cload(1, c1h, c1l);
         if (NCV >= 2) cload(2, c2h, c2l);
         if (NCV >= 3) cload(3, c3h, c3l);
         if (NCV >= 4) cload(4, c4h, c4l);
-        if (st + 1 < nsteps) fetch(st + 1);
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int buf, const v4u& ah, const v4u& al) {
         uint16_t* Ah = Ls[buf];
         uint16_t* Al = Ah + A_SZ;
         uint16_t* Ch = Al + A_SZ;
         uint16_t* Cl = Ch + C_SZ;
         const int r = f >> 2, c8 = 8 * (f & 3);
-        *reinterpret_cast<uint4*>(Ah + (sub * 32 + r) * KP + c8) = rah;
-        *reinterpret_cast<uint4*>(Al + (sub * 32 + r) * KP + c8) = ral;
-        auto cstore = [&](int it, const uint4& h, const uint4& l) {
+        *reinterpret_cast<v4u*>(Ah + (sub * 32 + r) * KP + c8) = ah;
+        *reinterpret_cast<v4u*>(Al + (sub * 32 + r) * KP + c8) = al;
+        // lanes with no vector in the partial round store theirs into the unused pitch padding
+        // (k 32..39) of some column, so every loaded value has an unconditional use and the
+        // loads stay where they are issued
+        auto cstore = [&](int it, const v4u& h, const v4u& l) {
             const int v = tid + NTHR * it;
-            if (it < NCV || v < NCV_R + NTHR * NCV) {
-                const int col = v >> 2, k8 = 8 * (v & 3);
-                *reinterpret_cast<uint4*>(Ch + col * KP + k8) = h;
-                *reinterpret_cast<uint4*>(Cl + col * KP + k8) = l;
-            }
+            int o;
+            if (it < NCV || v < NCV_R + NTHR * NCV)
+                o = (v >> 2) * KP + 8 * (v & 3);
+            else
+                o = ((v - PAN * 4) % PAN) * KP + 32;
+            *reinterpret_cast<v4u*>(Ch + o) = h;
+            *reinterpret_cast<v4u*>(Cl + o) = l;
         };
         cstore(0, c0h, c0l);
         if (NCV >= 1) cstore(1, c1h, c1l);
@@ -423,14 +430,14 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const uint16_t* Ch = Al + A_SZ;
         const uint16_t* Cl = Ch + C_SZ;
         const int k8 = 8 * (lane >> 4);
-        // all fragments of the step first (one LDS wait), then the MFMAs product by
-        // product, so consecutive updates of one accumulator are 4*NBW MFMAs apart
-        bf8_t ah[4], al[4], bh[NBW], bl[NBW];
+        // hi fragments and C-lo first; A-lo is read after the hi*lo products are issued, into
+        // the registers C-lo frees, so consecutive updates of one accumulator stay 4*NBW
+        // MFMAs apart with one fragment set fewer live
+        bf8_t ah[4], bh[NBW], bl[NBW];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int row = wm * 64 + i * 16 + (lane & 15);
             ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ah + row * KP + k8));
-            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + row * KP + k8));
         }
 #pragma unroll
         for (int j = 0; j < NBW; ++j) {
@@ -446,26 +453,45 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        bf8_t al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = wm * 64 + i * 16 + (lane & 15);
+            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + row * KP + k8));
+        }
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
     };
-
+    // Steps run in pairs so the two A sets are named statically; the loop body has no
+    // branches around its loads and stores (loads past the last step re-read it, and their
+    // stores land in a buffer no later compute reads).
     if (nsteps > 0) {
-        fetch(0);
-        gload(0);
-        lstore(0);
+        gloadC(0);
+        gloadA(0, a0h, a0l);
+        lstore(0, a0h, a0l);
+        gloadA(1, a1h, a1l);
         __syncthreads();
-        int buf = 0;
-        for (int st = 0; st < nsteps; ++st) {
-            const bool more = st + 1 < nsteps;
-            if (more) gload(st + 1);
-            compute(buf);
-            if (more) lstore(buf ^ 1);
+        int st = 0;
+        // (scheduling fences keep the loads ahead of the MFMAs they are meant to overlap)
+        for (; st + 1 < nsteps; st += 2) {
+            gloadC(st + 1);
+            gloadA(st + 2, a0h, a0l);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(0);
+            __builtin_amdgcn_sched_barrier(0);
+            lstore(1, a1h, a1l);
             __syncthreads();
-            buf ^= 1;
+            gloadC(st + 2);
+            gloadA(st + 3, a1h, a1l);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(1);
+            __builtin_amdgcn_sched_barrier(0);
+            lstore(0, a0h, a0l);
+            __syncthreads();
         }
+        if (st < nsteps) compute(0);  // odd step count: the last step is in buffer 0
     }
     const size_t xq = static_cast<size_t>(q) * xrows * ldc;
 #pragma unroll
