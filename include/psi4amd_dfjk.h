@@ -57,6 +57,12 @@ enum dfjk_fitting {
     DFJK_FIT_EIGEN = 1     /* B = J^-1/2 (P|mn) with psi4's condition cut (libmints/matrix.cc:2370-2424) */
 };
 
+enum dfjk_subalgorithm {      /* psi4 SCF_SUBTYPE (DFHelper subalgo_, dfhelper.h:370-376; jk.h:1039 set_subalgo) */
+    DFJK_SUBALGO_AUTO = 0,        /* out of core when the memory budget is below B (+ Bw) + metric */
+    DFJK_SUBALGO_INCORE = 1,      /* B resident in HBM */
+    DFJK_SUBALGO_OUT_OF_CORE = 2  /* B in pinned host memory, streamed through the device per compute */
+};
+
 const char* dfjk_last_error(void);
 
 /* JK::build_JK(primary, aux, ...) for MEM_DF (libfock/jk.cc:206-230) -> MemDFJK ctor (MemDFJK.cc:56-64). */
@@ -73,6 +79,9 @@ int dfjk_set_condition(dfjk_handle* h, double condition);  /* MemDFJK::set_condi
 int dfjk_set_do_J(dfjk_handle* h, int do_J);               /* JK::set_do_J */
 int dfjk_set_do_K(dfjk_handle* h, int do_K);               /* JK::set_do_K */
 int dfjk_set_omp_nthread(dfjk_handle* h, int n);           /* JK::set_omp_nthread: host integral threads */
+int dfjk_set_do_wK(dfjk_handle* h, int do_wK);             /* JK::set_do_wK (jk.h:459) */
+int dfjk_set_subalgorithm(dfjk_handle* h, int subalgo);    /* enum dfjk_subalgorithm (DFHelper subalgo_) */
+int dfjk_set_omega(dfjk_handle* h, double omega);          /* JK::set_omega (jk.h:473): erf(omega r)/r */
 int dfjk_set_precision(dfjk_handle* h, int precision);     /* enum dfjk_precision (no psi4 counterpart) */
 int dfjk_set_fitting(dfjk_handle* h, int fitting);         /* enum dfjk_fitting */
 int dfjk_set_stream(dfjk_handle* h, void* hip_stream);     /* NULL: the handle's own stream */
@@ -89,6 +98,12 @@ int dfjk_set_B_dense(dfjk_handle* h, const double* B, int on_device);
 int dfjk_fill_synthetic(dfjk_handle* h, uint64_t seed);
 /* Read back B_Q for Q in [q0,q1) as dense [q1-q0][nbf][nbf] host array (tests). */
 int dfjk_get_B_dense(dfjk_handle* h, int q0, int q1, double* out);
+/* The attenuated tensor Bw = M (Q|erf(omega r)/r|mn) of range-separated exchange, fitted with
+ * the Coulomb metric factor M of B (J^-1 = M^T M).  With a basis it is built by
+ * dfjk_initialize when do_wK is set, or at the first wK compute after an omega change;
+ * tensor-level handles take it from the caller.  Same dense layout as dfjk_set_B_dense. */
+int dfjk_set_Bw_dense(dfjk_handle* h, const double* Bw, int on_device);
+int dfjk_get_Bw_dense(dfjk_handle* h, int q0, int q1, double* out);
 
 /* JK::compute (jk.cc:595-681) -> MemDFJK::compute_JK (MemDFJK.cc:97-111) -> DFHelper::build_JK.
  * n density pairs; Cr == NULL means lr_symmetric (C_right = C_left, jk.cc:597-602).
@@ -98,6 +113,15 @@ int dfjk_compute(dfjk_handle* h, int n, const double* const* Cl, const double* c
                  double* const* D, double* const* J, double* const* K, int accumulate);
 int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
                         double* const* D, double* const* J, double* const* K, int accumulate);
+/* JK::wK() of JK::compute with do_wK (jk.cc:382-423) -> DFHelper::compute_wK
+ * (dfhelper.cc:3378-3438), MemDFJK's non-combined path (wcombine off, MemDFJK.cc:81-90):
+ *   wK = sum_Q (B_Q Cl)(Bw_Q Cr)^T = sum_{PQ} ((P|mn) Cl) [J^-1]_PQ ((Q|erf(omega r)/r|mn) Cr)^T,
+ * FP64 whatever the precision setting; Cr == NULL means Cr = Cl.  Not symmetric in general.
+ * wK[i]: nbf x nbf, overwritten unless accumulate != 0. */
+int dfjk_compute_wK(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                    double* const* wK, int accumulate);
+int dfjk_compute_wK_device(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                           double* const* wK, int accumulate);
 /* Restrict the auxiliary index range used by compute (J and K are additive over Q). */
 int dfjk_set_aux_range(dfjk_handle* h, int q0, int q1);
 
@@ -120,6 +144,10 @@ int dfjk_get_timings(dfjk_handle* h, double* ms_j, double* ms_ht, double* ms_kge
                      int* n_kgemm);
 /* Sizes: nbf, naux, stored tile pairs, significant function pairs. */
 int dfjk_get_info(dfjk_handle* h, int* nbf, int* naux, int* ntile_pairs, int64_t* nsig_pairs);
+
+/* Whether initialize chose the out-of-core subalgorithm (MemDFJK print_header "Algorithm: Core/Disk",
+ * MemDFJK.cc:124, from DFHelper::get_AO_core). */
+int dfjk_get_out_of_core(dfjk_handle* h, int* ooc);
 
 /* JK::finalize (jk.cc) -> postiterations: frees the workspace, keeps B. */
 int dfjk_finalize(dfjk_handle* h);

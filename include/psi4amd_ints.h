@@ -72,6 +72,14 @@ int p4a_ints_metric_grad(const p4a_basis* aux, const double* W, double* g_aux, i
 int p4a_ints_schwarz(const p4a_basis* prim, double* fun_max, double* shell_max, double* max_val, int nthreads);
 /* Full (mn|ls), nbf^4, for brute-force checks on tiny systems. */
 int p4a_ints_four_index(const p4a_basis* b, double* out, int nthreads);
+/* The same integrals over the attenuated operator erf(omega r)/r, omega > 0
+ * (IntegralFactory::erf_eri, libmints/integral.h:498, integral.cc:228): the metric,
+ * the three-index (P|mn) with the layout arguments of p4a_ints_three_index, and the
+ * four-index brute-force set. */
+int p4a_ints_metric_erf(const p4a_basis* aux, double omega, double* J, int nthreads);
+int p4a_ints_three_index_erf(const p4a_basis* prim, const p4a_basis* aux, double omega, const int64_t* dst,
+                             int64_t p_stride, const uint8_t* shell_mask, double* out, int nthreads);
+int p4a_ints_four_index_erf(const p4a_basis* b, double omega, double* out, int nthreads);
 /* Boys function F_0..F_nmax at T. */
 int p4a_ints_boys(int nmax, double T, double* F);
 

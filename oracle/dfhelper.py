@@ -199,6 +199,16 @@ def compute_K(Ppq, sp: Sparsity, Cl, Cr, K, lr_symmetric):
     K += T1 @ T2.T
 
 
+def compute_wK(m1Ppq, wPpq, sp: Sparsity, Cl, Cr, wK):
+    """dfhelper.cc:3378-3438 (non-combined wK): T1 from J^-1 (Q|mn) and C_left, T2 from
+    the raw attenuated (Q|w|mn) and C_right, wK += T1 T2^T (accumulates)."""
+    if Cl.shape[1] == 0:
+        return
+    T1 = first_transform_pQq(m1Ppq, sp, Cl).reshape(sp.nbf, -1)
+    T2 = first_transform_pQq(wPpq, sp, Cr).reshape(sp.nbf, -1)
+    wK += T1 @ T2.T
+
+
 def build_JK(Ppq, sp: Sparsity, C_left, C_right=None, do_J=True, do_K=True):
     """JK::compute (libfock/jk.cc:595-681) + MemDFJK::compute_JK (MemDFJK.cc:97-111)
     + DFHelper::compute_JK (dfhelper.cc:3044-3161) with one Q block.
@@ -237,16 +247,25 @@ class MemDFJKOracle:
     and the contractions run on that tensor, so J/K parity of the device path
     can be checked on identical B."""
 
-    def __init__(self, primary, aux, cutoff=1e-12, condition=1e-10, nthreads=None, fit="eigen", B=None):
+    def __init__(self, primary, aux, cutoff=1e-12, condition=1e-10, nthreads=None, fit="eigen", B=None,
+                 omega=None, Bw=None):
         from psi4_amd import ints  # the Libint2 stand-in (host integral engine)
         self.primary, self.aux = primary, aux
         self.cutoff, self.condition = cutoff, condition
         self.nbf, self.naux = primary.nbf(), aux.nbf()
+        self.omega = omega
         fm, sm, mx = ints.schwarz(primary, nthreads)
         self.sp = prepare_sparsity(fm, sm, mx, cutoff, self.naux)
+        self.m1Ppq = self.wPpq = None
         if B is not None:
             self.Ppq = ppq_from_dense(np.asarray(B), self.sp)
             self.metric = self.metp = self.Pmn = None
+            if Bw is not None:
+                # given fitted pairs B = M (P|mn), Bw = M (P|w|mn) with J^-1 = M^T M, the two
+                # sides of compute_wK carry one factor M each: sum_Q (B_Q Cl)(Bw_Q Cr)^T equals
+                # psi4's (J^-1 (P|mn) Cl)((P|w|mn) Cr)^T
+                self.m1Ppq = self.Ppq
+                self.wPpq = ppq_from_dense(np.asarray(Bw), self.sp)
             return
         self.metric = ints.metric(aux, nthreads)
         if fit == "eigen":
@@ -262,6 +281,17 @@ class MemDFJKOracle:
         # dropped pairs contribute nothing (psi4 never computes them)
         self.Pmn[:, ~self.sp.fun_mask] = 0.0
         self.Ppq = build_ppq(self.Pmn, self.metp, self.sp)
+        if omega:
+            # prepare_AO_wK_core (dfhelper.cc:589-699, non-combined): J^-1 (Q|mn) with the
+            # metric power wmpower_ = -1 (dfhelper.h:357) and the raw attenuated tensor
+            if fit == "eigen":
+                met1, _, _ = metric_power(self.metric, -1.0, condition)
+            else:
+                met1 = self.metp.T @ self.metp
+            self.m1Ppq = build_ppq(self.Pmn, met1, self.sp)
+            W = ints.three_index(primary, aux, nthreads, omega=omega)
+            W[:, ~self.sp.fun_mask] = 0.0
+            self.wPpq = build_ppq(W, np.eye(self.naux), self.sp)
 
     def memory_estimate(self, nthreads=1):
         return memory_estimate(self.sp, self.aux.max_nfunc(), nthreads)
@@ -271,6 +301,18 @@ class MemDFJKOracle:
 
     def compute(self, C_left, C_right=None, do_J=True, do_K=True):
         return build_JK(self.Ppq, self.sp, C_left, C_right, do_J, do_K)
+
+    def compute_wK(self, C_left, C_right=None):
+        """wK of every pair (DFHelper::build_JK -> compute_wK, dfhelper.cc:3032-3035)."""
+        if self.wPpq is None:
+            raise ValueError("oracle built without omega / Bw")
+        C_right = C_left if C_right is None else C_right
+        out = []
+        for Cl, Cr in zip(C_left, C_right):
+            wK = np.zeros((self.nbf, self.nbf))
+            compute_wK(self.m1Ppq, self.wPpq, self.sp, Cl, Cr, wK)
+            out.append(wK)
+        return out
 
 
 def ppq_from_dense(B, sp: Sparsity):

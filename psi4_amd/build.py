@@ -49,11 +49,34 @@ def build_ints(verbose=False):
     return out
 
 
+DFJK_SOURCES = ("kernels.hip", "kernels_k.hip", "kernels_ints.hip", "kernels_ints_l0.hip", "kernels_ints_l1.hip",
+                "kernels_ints_l2.hip", "kernels_ints_l3.hip", "kernels_ints_l4.hip", "dfjk.hip")
+
+
 def build_dfjk(verbose=False):
-    srcs = [os.path.join(CSRC, "dfjk", f) for f in ("kernels.hip", "kernels_k.hip", "dfjk.hip")]
+    """Each HIP source to an object in parallel (the integral classes dominate the build), then link."""
+    objdir = os.path.join(LIB, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    jobs = int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))
+    procs, objs = [], []
+    for f in DFJK_SOURCES:
+        obj = os.path.join(objdir, f.replace(".hip", ".o"))
+        objs.append(obj)
+        cmd = [hipcc, *HIPCC_FLAGS, "-c", f"-I{INC}", os.path.join(CSRC, "dfjk", f), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        while sum(p.poll() is None for p, _ in procs) >= max(1, jobs):
+            procs[[p.poll() is None for p, _ in procs].index(True)][0].wait()
+        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True), cmd))
+    for p, cmd in procs:
+        out_txt = p.communicate()[0]
+        if p.returncode != 0:
+            sys.stderr.write(out_txt)
+            raise RuntimeError(f"build step failed: {cmd[0]} ... {cmd[-3]}")
     out = os.path.join(LIB, "libpsi4amd_dfjk.so")
-    _run([os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, "-shared", f"-I{INC}", *srcs,
-          f"-L{LIB}", "-lp4a_ints", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"], verbose)
+    _run([hipcc, *HIPCC_FLAGS, "-shared", *objs, f"-L{LIB}", "-lp4a_ints", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"],
+         verbose)
     os.replace(out + ".tmp", out)
     return out
 

@@ -22,6 +22,7 @@
 #include "../ints/ints.hpp"
 #include "../ints/hostlinalg.hpp"
 #include "kernels.hpp"
+#include "kernels_ints.hpp"
 #include "kernels_k.hpp"
 #include "psi4amd_dfjk.h"
 
@@ -147,6 +148,8 @@ struct dfjk_handle {
     size_t memory_doubles = 0;  // 0: no explicit budget
     double cutoff = 1e-12, condition = 1e-10;
     bool do_J = true, do_K = true;
+    bool do_wK = false;
+    double omega = 0.0;
     int nthreads = 1;
     int precision = DFJK_PREC_FP64;
     int fitting = DFJK_FIT_CHOLESKY;
@@ -172,6 +175,18 @@ struct dfjk_handle {
     // resident tensor and its lower-precision copies
     DevBuf dB, dB32, dBh, dBl, dBhT, dBlT;
     DevBuf dZero;  // one zero TILE x TILE FP64 block, read in place of absent blocks
+    // out of core (DFHelper AO_core false, dfhelper.cc:216-260): B lives in pinned host memory
+    // and each compute streams it through two device stage buffers, Q-block by Q-block
+    int subalgo = DFJK_SUBALGO_AUTO;
+    bool ooc = false;
+    double* hB = nullptr;               // pinned host B, b_doubles
+    hipStream_t cstream = nullptr;      // copy stream (default priority, ordered by events)
+    hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    DevBuf ws_stage[2], ws_stage32, ws_stagebf;
+    // range-separated exchange: Bw = M (P|erf(omega r)/r|mn), same layout and fit as B
+    DevBuf dBw;
+    double bw_omega = -1.0;         // omega dBw was built for (< 0: none)
+    std::vector<double> fit_M;      // host metric factor, kept for rebuilding Bw after an omega change
     // workspace
     DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G, ws_sign, ws_c32, ws_cbf, ws_cmap;
     DevBuf ws_ct, ws_red, ws_Y, ws_PC, ws_gam, ws_dc;
@@ -328,38 +343,239 @@ std::vector<double> metric_factor(dfjk_handle* h) {
     return M;
 }
 
-// Upload the host A = (P|mn) (tiled layout) and fit B = M A on the device, in column chunks.
-void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vector<double>& A) {
+// Upload the host A = (P|mn) (tiled layout) and fit M A on the device into dst, in column chunks.
+// With host_dst, the fitted columns go to pinned host memory (out-of-core B) instead.
+void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vector<double>& A, DevBuf& dst,
+                   double* host_dst = nullptr) {
     const int na = h->naux;
     const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
     double* dM = h->ws_G.get<double>(static_cast<size_t>(na) * na);
     hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
-    double* dBp = h->dB.get<double>(b_doubles(h));
     // chunk of columns so that the staging buffer stays <= ~1 GiB
     size_t cw = std::max<size_t>(TILE2, (size_t(1) << 27) / std::max(1, na));
     cw = std::min(ncolB, cw / TILE2 * TILE2);
     double* dA = h->ws_XL.get<double>(static_cast<size_t>(na) * cw);
+    double* dBp = host_dst ? h->ws_XR.get<double>(static_cast<size_t>(na) * cw) : dst.get<double>(b_doubles(h));
     for (size_t c0 = 0; c0 < ncolB; c0 += cw) {
         const size_t w = std::min(cw, ncolB - c0);
         hcheck(hipMemcpy2DAsync(dA, w * sizeof(double), A.data() + c0, ncolB * sizeof(double), w * sizeof(double), na,
                                 hipMemcpyHostToDevice, h->stream),
                "upload (P|mn)");
-        launch_gemm_nn_f64(dM, na, dA, static_cast<int>(w), dBp + c0, static_cast<int>(ncolB), na,
-                           static_cast<int>(w), na, 1.0, 0.0, h->stream);
+        if (host_dst) {
+            launch_gemm_nn_f64(dM, na, dA, static_cast<int>(w), dBp, static_cast<int>(w), na, static_cast<int>(w), na,
+                               1.0, 0.0, h->stream);
+            hcheck(hipMemcpy2DAsync(host_dst + c0, ncolB * sizeof(double), dBp, w * sizeof(double),
+                                    w * sizeof(double), na, hipMemcpyDeviceToHost, h->stream),
+                   "download fitted B");
+        } else {
+            launch_gemm_nn_f64(dM, na, dA, static_cast<int>(w), dBp + c0, static_cast<int>(ncolB), na,
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream);
+        }
         hcheck(hipStreamSynchronize(h->stream), "fit");
     }
     h->ws_XL.release();
+    h->ws_XR.release();
     h->ws_G.release();
 }
 
-void initialize_basis(dfjk_handle* h) {
-    if (h->tiles.empty()) {
-        prepare_sparsity(h);
-        build_tiles(h);
+// ---------------------------------------------------------------- device (P|mn)
+// The three-index integrals on the device (kernels_ints*.hip) for bases with primary l <= 3
+// and auxiliary l <= 4: written into A [naux][ntp*TILE2] in the tiled layout (A zeroed here).
+// Returns false (nothing launched) when a shell is outside those classes, when A and B
+// together do not fit the device, out of core, or with PSI4AMD_HOST_INTS set; the caller
+// then uses the host engine.
+std::vector<int64_t> tile_dst(const dfjk_handle* h);
+
+bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
+    if (std::getenv("PSI4AMD_HOST_INTS") || !std::getenv("PSI4AMD_DEVICE_INTS")) return false;
+    if (h->ooc) return false;  // a device-resident (P|mn) would not respect the memory budget
+    const Basis& pb = h->prim;
+    const Basis& ab = h->aux;
+    for (const Shell& sh : pb.shells)
+        if (sh.l > 3) return false;
+    for (const Shell& sh : ab.shells)
+        if (sh.l > 4) return false;
+    const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
+    const size_t abytes = static_cast<size_t>(h->naux) * ncolB * sizeof(double);
+    size_t free_b = 0, total_b = 0;
+    hcheck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (abytes + (h->dB.p ? 0 : abytes) + (size_t(2) << 30) > free_b) return false;
+
+    auto shells = [](const Basis& b, std::vector<GpuShell>& sh, std::vector<double>& e, std::vector<double>& c) {
+        for (size_t i = 0; i < b.shells.size(); ++i) {
+            const Shell& x = b.shells[i];
+            GpuShell g;
+            g.l = x.l;
+            g.nprim = static_cast<int>(x.exps.size());
+            g.p0 = static_cast<int>(e.size());
+            g.foff = b.offset[i];
+            for (int d = 0; d < 3; ++d) g.A[d] = x.A[d];
+            sh.push_back(g);
+            e.insert(e.end(), x.exps.begin(), x.exps.end());
+            c.insert(c.end(), x.coefs.begin(), x.coefs.end());
+        }
+    };
+    std::vector<GpuShell> ps, as;
+    std::vector<double> pe, pc, ae, ac;
+    shells(pb, ps, pe, pc);
+    shells(ab, as, ae, ac);
+    // significant shell pairs (the host engine's M >= N set), higher l first, by class
+    const int np = static_cast<int>(pb.shells.size());
+    std::vector<std::vector<int2>> pcls(16);
+    for (int M = 0; M < np; ++M)
+        for (int N = 0; N <= M; ++N) {
+            if (!h->shell_mask[static_cast<size_t>(M) * np + N]) continue;
+            int X = M, Y = N;
+            if (pb.shells[X].l < pb.shells[Y].l) std::swap(X, Y);
+            pcls[pb.shells[X].l * 4 + pb.shells[Y].l].push_back(make_int2(X, Y));
+        }
+    std::vector<std::vector<int2>> acls(5);
+    for (size_t i = 0; i < ab.shells.size(); ++i)
+        for (int k = 0; k < ab.shells[i].nfunc(); ++k) acls[ab.shells[i].l].push_back(make_int2(static_cast<int>(i), k));
+    std::vector<int2> pall, aall;
+    std::vector<size_t> poff(17, 0), aoff(6, 0);
+    for (int c = 0; c < 16; ++c) {
+        poff[c] = pall.size();
+        pall.insert(pall.end(), pcls[c].begin(), pcls[c].end());
     }
-    upload_tiles(h);
-    std::vector<double> M = metric_factor(h);
-    // (P|mn) directly in the tiled layout: dst map per function pair
+    poff[16] = pall.size();
+    for (int c = 0; c < 5; ++c) {
+        aoff[c] = aall.size();
+        aall.insert(aall.end(), acls[c].begin(), acls[c].end());
+    }
+    aoff[5] = aall.size();
+    // solid-harmonic rows and the Boys grid of the host engine
+    std::vector<double> c2s;
+    ThreeIndexArgs a;
+    for (int l = 0; l <= 4; ++l) {
+        a.c2s_off[l] = static_cast<int>(c2s.size());
+        const std::vector<double>& t = cart2sph(l);
+        c2s.insert(c2s.end(), t.begin(), t.end());
+    }
+    int bnt = 0, bncol = 0;
+    double bh = 0.0, btmax = 0.0;
+    const double* btab = boys_grid(&bnt, &bncol, &bh, &btmax);
+    const std::vector<int64_t> dst = tile_dst(h);
+
+    DevBuf d_ps, d_as, d_pe, d_pc, d_ae, d_ac, d_c2s, d_boys, d_dst, d_pairs, d_afun;
+    auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+        b.get<char>(bytes);
+        hcheck(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "upload integral tables");
+    };
+    up(d_ps, ps.data(), ps.size() * sizeof(GpuShell));
+    up(d_as, as.data(), as.size() * sizeof(GpuShell));
+    up(d_pe, pe.data(), pe.size() * sizeof(double));
+    up(d_pc, pc.data(), pc.size() * sizeof(double));
+    up(d_ae, ae.data(), ae.size() * sizeof(double));
+    up(d_ac, ac.data(), ac.size() * sizeof(double));
+    up(d_c2s, c2s.data(), c2s.size() * sizeof(double));
+    up(d_boys, btab, static_cast<size_t>(bnt) * bncol * sizeof(double));
+    up(d_dst, dst.data(), dst.size() * sizeof(int64_t));
+    up(d_pairs, pall.data(), std::max<size_t>(1, pall.size()) * sizeof(int2));
+    up(d_afun, aall.data(), std::max<size_t>(1, aall.size()) * sizeof(int2));
+
+    double* A = Abuf.get<double>(static_cast<size_t>(h->naux) * ncolB);
+    hcheck(hipMemsetAsync(A, 0, abytes, h->stream), "memset A");
+    a.prim = d_ps.as<GpuShell>();
+    a.aux = d_as.as<GpuShell>();
+    a.pexp = d_pe.as<double>();
+    a.pcoef = d_pc.as<double>();
+    a.aexp = d_ae.as<double>();
+    a.acoef = d_ac.as<double>();
+    a.c2s = d_c2s.as<double>();
+    a.prim_pure = pb.shells.empty() ? 1 : (pb.shells[0].pure ? 1 : 0);
+    a.aux_pure = ab.shells.empty() ? 1 : (ab.shells[0].pure ? 1 : 0);
+    a.boys.table = d_boys.as<double>();
+    a.boys.ncol = bncol;
+    a.boys.h = bh;
+    a.boys.inv_h = 1.0 / bh;
+    a.boys.tmax = btmax;
+    a.dst = d_dst.as<int64_t>();
+    a.p_stride = static_cast<int64_t>(ncolB);
+    a.nbf = h->nbf;
+    a.out = A;
+    a.omega = omega;
+    const int2* dp = d_pairs.as<int2>();
+    const int2* da = d_afun.as<int2>();
+    for (int la = 0; la <= 3; ++la)
+        for (int lb = 0; lb <= la; ++lb) {
+            const int c = la * 4 + lb;
+            const int npair = static_cast<int>(poff[c + 1] - poff[c]);
+            if (!npair) continue;
+            for (int lc = 0; lc <= 4; ++lc) {
+                const int nf = static_cast<int>(aoff[lc + 1] - aoff[lc]);
+                if (!nf) continue;
+                launch_three_index_class(a, la, lb, lc, dp + poff[c], npair, da + aoff[lc], nf, h->stream);
+            }
+        }
+    hcheck(hipStreamSynchronize(h->stream), "three-index integrals");
+    return true;
+}
+
+// Fit B = M A from a device-resident A [naux][ncolB] into dst, or into pinned host memory.
+void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A, DevBuf& dst,
+                  double* host_dst = nullptr) {
+    const int na = h->naux;
+    const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
+    double* dM = h->ws_G.get<double>(static_cast<size_t>(na) * na);
+    hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
+    if (!host_dst) {
+        double* dBp = dst.get<double>(b_doubles(h));
+        size_t cw = std::max<size_t>(TILE2, (size_t(1) << 30) / (sizeof(double) * std::max(1, na)));
+        cw = std::min(ncolB, cw / TILE2 * TILE2);
+        for (size_t c0 = 0; c0 < ncolB; c0 += cw) {
+            const size_t w = std::min(cw, ncolB - c0);
+            launch_gemm_nn_f64(dM, na, A + c0, static_cast<int>(ncolB), dBp + c0, static_cast<int>(ncolB), na,
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream);
+        }
+    } else {
+        size_t cw = std::max<size_t>(TILE2, (size_t(1) << 27) / std::max(1, na));
+        cw = std::min(ncolB, cw / TILE2 * TILE2);
+        double* tmp = h->ws_XR.get<double>(static_cast<size_t>(na) * cw);
+        for (size_t c0 = 0; c0 < ncolB; c0 += cw) {
+            const size_t w = std::min(cw, ncolB - c0);
+            launch_gemm_nn_f64(dM, na, A + c0, static_cast<int>(ncolB), tmp, static_cast<int>(w), na,
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream);
+            hcheck(hipMemcpy2DAsync(host_dst + c0, ncolB * sizeof(double), tmp, w * sizeof(double),
+                                    w * sizeof(double), na, hipMemcpyDeviceToHost, h->stream),
+                   "download fitted B");
+            hcheck(hipStreamSynchronize(h->stream), "fit");
+        }
+        h->ws_XR.release();
+    }
+    hcheck(hipStreamSynchronize(h->stream), "fit");
+    h->ws_G.release();
+}
+
+// Resident bytes of the in-core algorithm (B, Bw when wK is on, the metric), against which
+// AUTO decides the subalgorithm the way DFHelper::AO_core does with required_core_size_.
+size_t resident_doubles(const dfjk_handle* h) {
+    return b_doubles(h) * (h->do_wK ? 2 : 1) + static_cast<size_t>(h->naux) * h->naux;
+}
+
+void choose_subalgorithm(dfjk_handle* h) {
+    if (h->subalgo == DFJK_SUBALGO_OUT_OF_CORE)
+        h->ooc = true;
+    else if (h->subalgo == DFJK_SUBALGO_INCORE)
+        h->ooc = false;
+    else
+        h->ooc = h->memory_doubles && h->memory_doubles < resident_doubles(h);
+    if (h->ooc && h->do_wK)  // dfhelper.cc:201-205
+        fail(DFJK_ERR_STATE, "DFJK: not equipped to do wK out of core; supply more memory or disable wK");
+    if (h->ooc && h->tensor_mode) fail(DFJK_ERR_STATE, "DFJK: tensor-level handles are in core only");
+    if (h->ooc) {
+        if (!h->hB) hcheck(hipHostMalloc(reinterpret_cast<void**>(&h->hB), b_doubles(h) * sizeof(double), 0),
+                           "hipHostMalloc B");
+        if (!h->cstream) hcheck(hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking), "copy stream");
+        for (int i = 0; i < 2; ++i) {
+            if (!h->ev_copied[i]) hcheck(hipEventCreateWithFlags(&h->ev_copied[i], hipEventDisableTiming), "event");
+            if (!h->ev_free[i]) hcheck(hipEventCreateWithFlags(&h->ev_free[i], hipEventDisableTiming), "event");
+        }
+    }
+}
+
+// Destination of every significant function pair (m,n) in the tiled layout (-1: not stored).
+std::vector<int64_t> tile_dst(const dfjk_handle* h) {
     const int n = h->nbf;
     std::vector<int64_t> dst(static_cast<size_t>(n) * n, -1);
     for (int m = 0; m < n; ++m)
@@ -370,11 +586,55 @@ void initialize_basis(dfjk_handle* h) {
             const int t = h->tidx[I * h->nT + J];
             dst[static_cast<size_t>(m) * n + q] = static_cast<int64_t>(t) * TILE2 + (m % TILE) * TILE + q % TILE;
         }
-    const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
-    std::vector<double> A(static_cast<size_t>(h->naux) * ncolB, 0.0);
-    three_index(h->prim, h->aux, dst.data(), static_cast<int64_t>(ncolB), h->shell_mask.data(), A.data(),
-                h->nthreads);
-    fit_on_device(h, M, A);
+    return dst;
+}
+
+// (P|erf(omega r)/r|mn) on the host, fitted with the Coulomb metric factor into dBw.  psi4 keeps
+// the raw attenuated tensor and contracts the full inverse metric into the other side
+// (dfhelper.cc:589-699, 3378-3438); with J^-1 = M^T M both sides carry one factor M here.
+void build_Bw(dfjk_handle* h) {
+    if (h->tensor_mode) fail(DFJK_ERR_STATE, "wK needs the attenuated tensor: dfjk_set_Bw_dense");
+    if (!(h->omega > 0.0)) fail(DFJK_ERR_ARG, "wK needs omega > 0 (dfjk_set_omega)");
+    if (h->fit_M.empty()) h->fit_M = metric_factor(h);
+    if (gpu_three_index(h, h->omega, h->ws_XL)) {
+        fit_device_A(h, h->fit_M, h->ws_XL.as<double>(), h->dBw);
+        h->ws_XL.release();
+    } else {
+        const std::vector<int64_t> dst = tile_dst(h);
+        const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
+        std::vector<double> A(static_cast<size_t>(h->naux) * ncolB, 0.0);
+        three_index(h->prim, h->aux, dst.data(), static_cast<int64_t>(ncolB), h->shell_mask.data(), A.data(),
+                    h->nthreads, h->omega);
+        fit_on_device(h, h->fit_M, A, h->dBw);
+    }
+    h->bw_omega = h->omega;
+}
+
+void initialize_basis(dfjk_handle* h) {
+    if (h->tiles.empty()) {
+        prepare_sparsity(h);
+        build_tiles(h);
+    }
+    upload_tiles(h);
+    std::vector<double> M = metric_factor(h);
+    choose_subalgorithm(h);
+    // (P|mn) directly in the tiled layout: on the device when the classes and memory allow,
+    // else on the host
+    if (gpu_three_index(h, 0.0, h->ws_XL)) {
+        fit_device_A(h, M, h->ws_XL.as<double>(), h->dB, h->ooc ? h->hB : nullptr);
+        h->ws_XL.release();
+    } else {
+        const std::vector<int64_t> dst = tile_dst(h);
+        const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
+        std::vector<double> A(static_cast<size_t>(h->naux) * ncolB, 0.0);
+        three_index(h->prim, h->aux, dst.data(), static_cast<int64_t>(ncolB), h->shell_mask.data(), A.data(),
+                    h->nthreads);
+        fit_on_device(h, M, A, h->dB, h->ooc ? h->hB : nullptr);
+    }
+    if (h->do_wK) {
+        h->fit_M = std::move(M);
+        build_Bw(h);
+    }
 }
 
 // ---------------------------------------------------------------- compute
@@ -517,88 +777,93 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         tm.end(T_OTHER, e0);
     }
 
-    // J (dfhelper.cc:3162-3285): d_Q = <B_Q, D>, J = sum_Q B_Q d_Q, FP64 accumulation.  The
-    // tensor is read from the copy that matches the precision setting: FP64, the FP32 copy,
-    // or hi + lo of the BF16 pair (half the bytes of the FP64 pass for the mixed builds).
-    if (h->do_J && dJ) {
-        JSource js;
-        if (h->precision == DFJK_PREC_FP32) {
-            js.kind = JSource::F32;
-            js.a = ensure_B32(h);
-        } else if (h->precision == DFJK_PREC_BF16X3) {
-            ensure_Bbf(h);
-            js.kind = JSource::BF2;
-            js.a = h->dBh.p;
-            js.b = h->dBl.p;
-        } else {
-            js.a = h->dB.p;
-        }
+    // J (dfhelper.cc:3162-3285) and K (dfhelper.cc:2162-2186, 3350-3377) over Q-blocks.
+    // J_Q-block: d_Q = <B_Q, D>, J += sum_Q B_Q d_Q (additive over Q, FP64 accumulation);
+    // K_Q-block: X_Q = B_Q C, K += sum_Q X_Q X'_Q^T.  In core the blocks are views of the
+    // resident tensor (or of its FP32 / BF16 copies in the mixed modes); out of core each
+    // block is copied from pinned host memory into one of two stage buffers on the copy
+    // stream while the previous block computes, and converted there for the mixed modes.
+    const bool doJ = h->do_J && dJ;
+    const bool doK = h->do_K && dK;
+    if (!doJ && !doK) return;
+    const int prec = h->precision;
+    const size_t qs = static_cast<size_t>(h->ntp) * TILE2;  // elements per Q
+    const size_t xq = static_cast<size_t>(xrows) * ldc;      // X elements per Q
+    const size_t esz = prec == DFJK_PREC_FP64 ? 8 : 4;       // FP64, FP32, or BF16 hi+lo
+    size_t budget = g_x_budget;                              // bytes of X workspace
+    if (h->memory_doubles) {
+        const size_t resident =
+            (h->ooc ? 0 : b_doubles(h)) * sizeof(double) + static_cast<size_t>(h->naux) * h->naux * sizeof(double);
+        const size_t mem = h->memory_doubles * sizeof(double);
+        budget = mem > resident ? std::min(budget, mem - resident) : 0;
+    }
+    int qblk = nq_total;
+    if (doK) {
+        const size_t per_q = xq * esz * (general ? 2 : 1);
+        qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
+        qblk = std::min(qblk, 65535);
+    }
+    if (h->ooc) {
+        // two FP64 stage buffers (plus their converted copies) share the budget with X
+        const size_t per_q = qs * (sizeof(double) + (prec == DFJK_PREC_FP64 ? 0 : 2 * sizeof(float))) * 2;
+        const size_t stage_budget = std::max<size_t>(per_q, std::min<size_t>(size_t(8) << 30, budget / 2));
+        qblk = std::min<int>(qblk, static_cast<int>(std::max<size_t>(1, stage_budget / per_q)));
+    }
+    const int nblocks = (nq_total + qblk - 1) / qblk;
+    qblk = (nq_total + nblocks - 1) / nblocks;
+
+    // J workspace: the symmetrised tile-packed density and per-block partials
+    double* W = nullptr;
+    double *dpart = nullptr, *dvec = nullptr, *Jpart = nullptr;
+    int jchunk = 0, jnchunk = 0;
+    if (doJ) {
         hipEvent_t e0 = tm.begin();
-        double* W = h->ws_W.get<double>(static_cast<size_t>(h->ntp) * TILE2);
-        double* dpart = h->ws_dpart.get<double>(static_cast<size_t>(h->ntp) * h->naux);
-        double* dvec = h->ws_d.get<double>(h->naux);
+        W = h->ws_W.get<double>(qs);
         launch_pack_w(g, Dm, nbf, W, s);
-        if (q0 > 0 || q1 < h->naux)
-            hcheck(hipMemsetAsync(dpart, 0, sizeof(double) * static_cast<size_t>(h->ntp) * h->naux, s), "memset");
-        launch_j1(g, js, W, h->naux, q0, q1, dpart, s);
-        launch_j1_reduce(g, dpart, h->naux, dvec, s);
-        const int want_wg = 4096;
-        int nchunk = std::max(1, std::min(nq_total, (want_wg + h->ntp - 1) / h->ntp));
-        const int qchunk = (h->naux + nchunk - 1) / nchunk;
-        nchunk = (h->naux + qchunk - 1) / qchunk;
-        double* Jpart = h->ws_Jpart.get<double>(static_cast<size_t>(nchunk) * h->ntp * TILE2);
-        launch_j2(g, js, dvec, h->naux, qchunk, Jpart, s);
-        launch_j_unpack(g, Jpart, nchunk, dJ, nbf, 1.0, accumulate, s);
         tm.end(T_J, e0);
+        dvec = h->ws_d.get<double>(h->naux);  // d_Q at its absolute Q (the adjoint reads it afterwards)
+        if (h->ooc) {
+            dpart = h->ws_dpart.get<double>(static_cast<size_t>(h->ntp) * qblk);
+            const int want_wg = 4096;
+            jnchunk = std::max(1, std::min(qblk, (want_wg + h->ntp - 1) / h->ntp));
+            jchunk = (qblk + jnchunk - 1) / jnchunk;
+            jnchunk = (qblk + jchunk - 1) / jchunk;
+            Jpart = h->ws_Jpart.get<double>(static_cast<size_t>(jnchunk) * qs);
+        }
     }
 
-    // K (dfhelper.cc:2162-2186, 3350-3377): X_Q = B_Q C, K = sum_Q X_Q X'_Q^T
-    if (h->do_K && dK) {
-        const int prec = h->precision;
-        const size_t xq = static_cast<size_t>(xrows) * ldc;  // elements per Q
-        const size_t esz = prec == DFJK_PREC_FP64 ? 8 : 4;    // FP64, FP32, or BF16 hi+lo
-        size_t budget = g_x_budget;                           // bytes of X workspace
-        if (h->memory_doubles) {
-            const size_t resident = (b_doubles(h) + static_cast<size_t>(h->naux) * h->naux) * sizeof(double);
-            const size_t mem = h->memory_doubles * sizeof(double);
-            budget = mem > resident ? std::min(budget, mem - resident) : 0;
-        }
-        const size_t per_q = xq * esz * (general ? 2 : 1);
-        int qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
-        qblk = std::min(qblk, 65535);
-        const int nblocks = (nq_total + qblk - 1) / qblk;
-        qblk = (nq_total + nblocks - 1) / nblocks;
-        const bool upper = sym || cross;  // symmetric K: contract the upper block-triangle only
-        const int ntile = upper ? h->ntile_sym : h->ntile_all;
-        const int2* otiles = upper ? h->d_otiles_sym.as<int2>() : h->d_otiles_all.as<int2>();
-        const int nsplit = choose_nsplit(ntile, qblk);
-        double* P = h->ws_P.get<double>(static_cast<size_t>(nsplit) * ntile * 16384);
-        // contraction passes over k: (kbeg, kend, alpha, column offsets of the two operands)
-        struct Pass {
-            int kbeg, kend;
-            double alpha;
-            size_t offl, offr;
-        };
-        // padding columns of X are zero: each pass stops at the contraction's k-step after
-        // the last real column of its group
-        const int kstep = prec == DFJK_PREC_BF16X3 ? 32 : 16;
-        std::vector<Pass> passes;
-        if (signed_sym) {
-            if (!plus.empty()) passes.push_back({0, round_up(static_cast<int>(plus.size()), kstep), 1.0, 0, 0});
-            passes.push_back({kplus, kplus + round_up(static_cast<int>(minus.size()), kstep), -1.0, 0, 0});
-        } else if (cross) {
-            const int kc = round_up(half, kstep);
-            passes.push_back({0, kc, 1.0, 0, static_cast<size_t>(wcross)});
-            passes.push_back({0, kc, 1.0, static_cast<size_t>(wcross), 0});
-        } else {
-            passes.push_back({0, std::min(ldc, round_up(ncol, kstep)), 1.0, 0, 0});
-        }
-
-        const float *B32 = nullptr, *c32l = nullptr, *c32r = nullptr;
-        uint16_t *cth_l = nullptr, *ctl_l = nullptr, *cth_r = nullptr, *ctl_r = nullptr;
+    // K set-up: output tiles, Q-splits, contraction passes, packed C
+    const bool upper = sym || cross;  // symmetric K: contract the upper block-triangle only
+    const int ntile = upper ? h->ntile_sym : h->ntile_all;
+    const int2* otiles = upper ? h->d_otiles_sym.as<int2>() : h->d_otiles_all.as<int2>();
+    const int nsplit = doK ? choose_nsplit(ntile, qblk) : 1;
+    double* P = doK ? h->ws_P.get<double>(static_cast<size_t>(nsplit) * ntile * 16384) : nullptr;
+    // contraction passes over k: (kbeg, kend, alpha, column offsets of the two operands)
+    struct Pass {
+        int kbeg, kend;
+        double alpha;
+        size_t offl, offr;
+    };
+    // padding columns of X are zero: each pass stops at the contraction's k-step after
+    // the last real column of its group
+    const int kstep = prec == DFJK_PREC_BF16X3 ? 32 : 16;
+    std::vector<Pass> passes;
+    if (signed_sym) {
+        if (!plus.empty()) passes.push_back({0, round_up(static_cast<int>(plus.size()), kstep), 1.0, 0, 0});
+        passes.push_back({kplus, kplus + round_up(static_cast<int>(minus.size()), kstep), -1.0, 0, 0});
+    } else if (cross) {
+        const int kc = round_up(half, kstep);
+        passes.push_back({0, kc, 1.0, 0, static_cast<size_t>(wcross)});
+        passes.push_back({0, kc, 1.0, static_cast<size_t>(wcross), 0});
+    } else {
+        passes.push_back({0, std::min(ldc, round_up(ncol, kstep)), 1.0, 0, 0});
+    }
+    const float *c32l = nullptr, *c32r = nullptr;
+    uint16_t *cth_l = nullptr, *ctl_l = nullptr, *cth_r = nullptr, *ctl_r = nullptr;
+    char *XL = nullptr, *XR = nullptr;
+    if (doK) {
         const size_t csz = static_cast<size_t>(nbfp) * ldc;
         if (prec == DFJK_PREC_FP32) {
-            B32 = ensure_B32(h);
             float* a = h->ws_c32.get<float>(2 * csz);
             launch_pack_cmap_f32(dCl, ncol, nbf, dmap, a, nbfp, ldc, s);
             c32l = c32r = a;
@@ -607,8 +872,6 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                 c32r = a + csz;
             }
         } else if (prec == DFJK_PREC_BF16X3) {
-            ensure_Bbf(h);
-            ensure_BbfT(h);
             uint16_t* a = h->ws_cbf.get<uint16_t>(4 * csz);
             cth_l = a;
             ctl_l = a + csz;
@@ -621,34 +884,152 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                 launch_pack_cT_bf(dCr, ncol, nbf, dmap, cth_r, ctl_r, nbfp, ldc, s);
             }
         }
-        char* XL = h->ws_XL.get<char>(xq * esz * qblk);
-        char* XR = general ? h->ws_XR.get<char>(xq * esz * qblk) : XL;
+        XL = h->ws_XL.get<char>(xq * esz * qblk);
+        XR = general ? h->ws_XR.get<char>(xq * esz * qblk) : XL;
+    }
 
-        for (int blk = 0; blk < nblocks; ++blk) {
-            const int qa = q0 + blk * qblk;
-            const int nq = std::min(qblk, q1 - qa);
+    // resident sources (in core)
+    if (!h->ooc) {
+        if (prec == DFJK_PREC_FP32) ensure_B32(h);
+        if (prec == DFJK_PREC_BF16X3) {
+            ensure_Bbf(h);
+            if (doK) ensure_BbfT(h);
+        }
+    }
+    // In core, J is one pass over the whole range (two streaming kernels over the resident
+    // tensor, or its lower-precision copy); out of core it is accumulated block by block below.
+    if (doJ && !h->ooc) {
+        JSource js;
+        if (prec == DFJK_PREC_FP32) {
+            js.kind = JSource::F32;
+            js.a = h->dB32.p;
+        } else if (prec == DFJK_PREC_BF16X3) {
+            js.kind = JSource::BF2;
+            js.a = h->dBh.p;
+            js.b = h->dBl.p;
+        } else {
+            js.a = h->dB.p;
+        }
+        hipEvent_t e0 = tm.begin();
+        double* dp = h->ws_dpart.get<double>(static_cast<size_t>(h->ntp) * h->naux);
+        if (q0 > 0 || q1 < h->naux)
+            hcheck(hipMemsetAsync(dp, 0, sizeof(double) * static_cast<size_t>(h->ntp) * h->naux, s), "memset");
+        launch_j1(g, js, W, h->naux, q0, q1, dp, s);
+        launch_j1_reduce(g, dp, h->naux, dvec, s);
+        const int want_wg = 4096;
+        int nchunk = std::max(1, std::min(nq_total, (want_wg + h->ntp - 1) / h->ntp));
+        const int qchunk = (h->naux + nchunk - 1) / nchunk;
+        nchunk = (h->naux + qchunk - 1) / qchunk;
+        double* Jp = h->ws_Jpart.get<double>(static_cast<size_t>(nchunk) * qs);
+        launch_j2(g, js, dvec, h->naux, qchunk, Jp, s);
+        launch_j_unpack(g, Jp, nchunk, dJ, nbf, 1.0, accumulate, s);
+        tm.end(T_J, e0);
+    }
+    auto stage_copy = [&](int blk) {  // out of core: host block -> stage buffer blk & 1
+        const int qa = q0 + blk * qblk;
+        const int nq = std::min(qblk, q1 - qa);
+        double* dst = h->ws_stage[blk & 1].get<double>(qs * qblk);
+        hcheck(hipStreamWaitEvent(h->cstream, h->ev_free[blk & 1], 0), "wait stage");
+        hcheck(hipMemcpyAsync(dst, h->hB + static_cast<size_t>(qa) * qs, qs * nq * sizeof(double),
+                              hipMemcpyHostToDevice, h->cstream),
+               "stage B");
+        hcheck(hipEventRecord(h->ev_copied[blk & 1], h->cstream), "record");
+    };
+    if (h->ooc) {
+        // both stage buffers start free
+        for (int i = 0; i < 2; ++i) hcheck(hipEventRecord(h->ev_free[i], s), "record");
+        stage_copy(0);
+    }
+
+    for (int blk = 0; blk < nblocks; ++blk) {
+        const int qa = q0 + blk * qblk;
+        const int nq = std::min(qblk, q1 - qa);
+        const size_t nel = qs * nq;
+        // this block's views of the tensor in every precision the kernels read
+        const double* b64;
+        const float* b32 = nullptr;
+        const uint16_t *bh = nullptr, *bl = nullptr, *bhT = nullptr, *blT = nullptr;
+        if (!h->ooc) {
+            const size_t off = static_cast<size_t>(qa) * qs;
+            b64 = h->dB.as<double>() + off;
+            if (prec == DFJK_PREC_FP32) b32 = h->dB32.as<float>() + off;
+            if (prec == DFJK_PREC_BF16X3) {
+                bh = h->dBh.as<uint16_t>() + off;
+                bl = h->dBl.as<uint16_t>() + off;
+                if (doK) {
+                    bhT = h->dBhT.as<uint16_t>() + off;
+                    blT = h->dBlT.as<uint16_t>() + off;
+                }
+            }
+        } else {
+            if (blk + 1 < nblocks) stage_copy(blk + 1);
+            hcheck(hipStreamWaitEvent(s, h->ev_copied[blk & 1], 0), "wait copy");
+            double* st = h->ws_stage[blk & 1].as<double>();
+            b64 = st;
+            hipEvent_t e0 = tm.begin();
+            if (prec == DFJK_PREC_FP32) {
+                float* c = h->ws_stage32.get<float>(qs * qblk);
+                launch_to_f32(st, c, nel, s);
+                b32 = c;
+            } else if (prec == DFJK_PREC_BF16X3) {
+                uint16_t* c = h->ws_stagebf.get<uint16_t>(4 * qs * qblk);
+                const size_t cap = qs * qblk;
+                launch_to_bf_split(st, c, c + cap, nel, s);
+                bh = c;
+                bl = c + cap;
+                if (doK) {
+                    launch_to_bf_split_t(st, c + 2 * cap, c + 3 * cap, nel, s);
+                    bhT = c + 2 * cap;
+                    blT = c + 3 * cap;
+                }
+            }
+            tm.end(T_OTHER, e0);
+        }
+
+        if (doJ && h->ooc) {
+            JSource js;
+            if (prec == DFJK_PREC_FP32) {
+                js.kind = JSource::F32;
+                js.a = b32;
+            } else if (prec == DFJK_PREC_BF16X3) {
+                js.kind = JSource::BF2;
+                js.a = bh;
+                js.b = bl;
+            } else {
+                js.a = b64;
+            }
+            hipEvent_t e0 = tm.begin();
+            const int nch = (nq + jchunk - 1) / jchunk;
+            launch_j1(g, js, W, nq, 0, nq, dpart, s);
+            launch_j1_reduce(g, dpart, nq, dvec + qa, s);
+            launch_j2(g, js, dvec + qa, nq, jchunk, Jpart, s);
+            launch_j_unpack(g, Jpart, nch, dJ, nbf, 1.0, (blk > 0 || accumulate) ? 1 : 0, s);
+            tm.end(T_J, e0);
+        }
+
+        if (doK) {
             const int nsp = std::min(nsplit, nq);
             hipEvent_t e0 = tm.begin();
             if (prec == DFJK_PREC_FP64) {
-                launch_ht2_f64(g, h->dB.as<double>(), qa, nq, cl, ldc, nbw, npanel, reinterpret_cast<double*>(XL),
-                               xrows, h->dZero.p, s);
+                launch_ht2_f64(g, b64, 0, nq, cl, ldc, nbw, npanel, reinterpret_cast<double*>(XL), xrows, h->dZero.p,
+                               s);
                 if (general)
-                    launch_ht2_f64(g, h->dB.as<double>(), qa, nq, cr, ldc, nbw, npanel, reinterpret_cast<double*>(XR),
-                                   xrows, h->dZero.p, s);
+                    launch_ht2_f64(g, b64, 0, nq, cr, ldc, nbw, npanel, reinterpret_cast<double*>(XR), xrows,
+                                   h->dZero.p, s);
             } else if (prec == DFJK_PREC_FP32) {
-                launch_ht2_f32(g, B32, qa, nq, c32l, ldc, nbw, npanel, reinterpret_cast<float*>(XL), xrows, h->dZero.p, s);
+                launch_ht2_f32(g, b32, 0, nq, c32l, ldc, nbw, npanel, reinterpret_cast<float*>(XL), xrows, h->dZero.p,
+                               s);
                 if (general)
-                    launch_ht2_f32(g, B32, qa, nq, c32r, ldc, nbw, npanel, reinterpret_cast<float*>(XR), xrows, h->dZero.p, s);
+                    launch_ht2_f32(g, b32, 0, nq, c32r, ldc, nbw, npanel, reinterpret_cast<float*>(XR), xrows,
+                                   h->dZero.p, s);
             } else {
                 uint16_t* xl = reinterpret_cast<uint16_t*>(XL);
                 uint16_t* xr = reinterpret_cast<uint16_t*>(XR);
-                launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), h->dBhT.as<uint16_t>(),
-                              h->dBlT.as<uint16_t>(), qa, nq, cth_l, ctl_l, ldc, nbw, npanel,
-                              xl, xl + xq * qblk, xrows, h->dZero.p, s);
+                launch_ht_bf3(g, bh, bl, bhT, blT, 0, nq, cth_l, ctl_l, ldc, nbw, npanel, xl, xl + xq * qblk, xrows,
+                              h->dZero.p, s);
                 if (general)
-                    launch_ht_bf3(g, h->dBh.as<uint16_t>(), h->dBl.as<uint16_t>(), h->dBhT.as<uint16_t>(),
-                                  h->dBlT.as<uint16_t>(), qa, nq, cth_r, ctl_r, ldc, nbw,
-                                  npanel, xr, xr + xq * qblk, xrows, h->dZero.p, s);
+                    launch_ht_bf3(g, bh, bl, bhT, blT, 0, nq, cth_r, ctl_r, ldc, nbw, npanel, xr, xr + xq * qblk,
+                                  xrows, h->dZero.p, s);
             }
             tm.end(T_HT, e0);
             h->n_ht += general ? 2 : 1;
@@ -680,11 +1061,82 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                        "memset");
             tm.end(T_KGEMM, e1);
         }
+        if (h->ooc) hcheck(hipEventRecord(h->ev_free[blk & 1], s), "record");
+    }
+    if (doK) {
         hipEvent_t e2 = tm.begin();
         launch_tile_reduce(P, nsplit, ntile, upper ? h->d_otile_of_sym.as<int>() : h->d_otile_of_all.as<int>(), h->nB,
                            upper ? 1 : 0, nbf, nbf, dK, nbf, 1.0, accumulate, s);
         tm.end(T_OTHER, e2);
     }
+}
+
+// wK of one pair (dfhelper.cc:3378-3438): wK = sum_Q (B_Q C_left)(Bw_Q C_right)^T with
+// Bw = M (Q|erf(omega r)/r|mn), so that wK_mn = sum_{PQ,i} ((P|mi) C_l) [J^-1]_PQ ((Q|w|ni) C_r) as
+// psi4 forms it from J^-1 (Q|mn) and the raw attenuated tensor.  FP64; every output tile is
+// contracted (the product of two different tensors is not symmetric).
+void compute_wk_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol, double* dwK, int accumulate) {
+    const TileGeom g = h->geom();
+    const hipStream_t s = h->stream;
+    const int q0 = h->aux_q0, q1 = h->aux_q1;
+    const int nq_total = q1 - q0;
+    const int nbf = h->nbf, nbfp = h->nbfp;
+    const int xrows = (h->nT + 3) / 4 * 128;
+    EventTimer& tm = h->timer;
+    int nbw, npanel, ldc;
+    ht_panel_shape(ncol, &nbw, &npanel, &ldc);
+    std::vector<int> cmap(ldc, -1);
+    for (int i = 0; i < ncol; ++i) cmap[i] = i;
+    int* dmap = h->ws_cmap.get<int>(ldc);
+    hcheck(hipMemcpy(dmap, cmap.data(), ldc * sizeof(int), hipMemcpyHostToDevice), "column map");
+    double* cl = h->ws_cl.get<double>(static_cast<size_t>(nbfp) * ldc);
+    launch_pack_cmap_f64(dCl, ncol, nbf, dmap, cl, nbfp, ldc, s);
+    double* cr = cl;
+    if (dCr) {
+        cr = h->ws_cr.get<double>(static_cast<size_t>(nbfp) * ldc);
+        launch_pack_cmap_f64(dCr, ncol, nbf, dmap, cr, nbfp, ldc, s);
+    }
+    const size_t xq = static_cast<size_t>(xrows) * ldc;
+    size_t budget = g_x_budget;
+    if (h->memory_doubles) {
+        const size_t resident = (2 * b_doubles(h) + static_cast<size_t>(h->naux) * h->naux) * sizeof(double);
+        const size_t mem = h->memory_doubles * sizeof(double);
+        budget = mem > resident ? std::min(budget, mem - resident) : 0;
+    }
+    const size_t per_q = 2 * xq * sizeof(double);
+    int qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
+    qblk = std::min(qblk, 65535);
+    const int nblocks = (nq_total + qblk - 1) / qblk;
+    qblk = (nq_total + nblocks - 1) / nblocks;
+    const int ntile = h->ntile_all;
+    const int2* otiles = h->d_otiles_all.as<int2>();
+    const int nsplit = choose_nsplit(ntile, qblk);
+    double* P = h->ws_P.get<double>(static_cast<size_t>(nsplit) * ntile * 16384);
+    const int kend = std::min(ldc, round_up(ncol, 16));
+    double* XL = h->ws_XL.get<double>(xq * qblk);
+    double* XR = h->ws_XR.get<double>(xq * qblk);
+    for (int blk = 0; blk < nblocks; ++blk) {
+        const int qa = q0 + blk * qblk;
+        const int nq = std::min(qblk, q1 - qa);
+        const int nsp = std::min(nsplit, nq);
+        hipEvent_t e0 = tm.begin();
+        launch_ht2_f64(g, h->dB.as<double>(), qa, nq, cl, ldc, nbw, npanel, XL, xrows, h->dZero.p, s);
+        launch_ht2_f64(g, h->dBw.as<double>(), qa, nq, cr, ldc, nbw, npanel, XR, xrows, h->dZero.p, s);
+        tm.end(T_HT, e0);
+        h->n_ht += 2;
+        hipEvent_t e1 = tm.begin();
+        launch_kgemm2_f64(XL, XR, xq, xrows, ldc, 0, kend, nq, nsp, otiles, ntile, P, 1.0, blk > 0 ? 1 : 0, 0, s);
+        h->n_kgemm += 1;
+        if (nsp < nsplit && blk == 0)
+            hcheck(hipMemsetAsync(P + static_cast<size_t>(nsp) * ntile * 16384, 0,
+                                  sizeof(double) * static_cast<size_t>(nsplit - nsp) * ntile * 16384, s),
+                   "memset");
+        tm.end(T_KGEMM, e1);
+    }
+    hipEvent_t e2 = tm.begin();
+    launch_tile_reduce(P, nsplit, ntile, h->d_otile_of_all.as<int>(), h->nB, 0, nbf, nbf, dwK, nbf, 1.0, accumulate,
+                       s);
+    tm.end(T_OTHER, e2);
 }
 
 void finish_timers(dfjk_handle* h) {
@@ -767,6 +1219,57 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
 }
 
 
+void compute_wk_impl(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                     double* const* wK, int accumulate, bool device) {
+    require_state(h, true);
+    if (n < 0 || (n > 0 && (!Cl || !ncol || !wK))) fail(DFJK_ERR_ARG, "JK: bad wK arguments");
+    if (h->tensor_mode) {
+        if (!h->dBw.p) fail(DFJK_ERR_STATE, "wK needs the attenuated tensor: dfjk_set_Bw_dense");
+    } else if (h->bw_omega != h->omega) {
+        build_Bw(h);
+    }
+    reset_timers(h);
+    const size_t nn = static_cast<size_t>(h->nbf) * h->nbf;
+    for (int i = 0; i < n; ++i) {
+        if (ncol[i] < 0) fail(DFJK_ERR_ARG, "JK: negative column count");
+        if (!Cl[i] && ncol[i] > 0) fail(DFJK_ERR_ARG, "JK: null C_left matrix");
+        if (Cr && !Cr[i] && ncol[i] > 0) fail(DFJK_ERR_ARG, "JK: null C_right matrix");
+        double* dW = wK[i];
+        if (!dW) continue;
+        if (ncol[i] == 0) {
+            if (!accumulate) {
+                if (device)
+                    hcheck(hipMemsetAsync(dW, 0, nn * sizeof(double), h->stream), "memset");
+                else
+                    std::memset(dW, 0, nn * sizeof(double));
+            }
+            continue;
+        }
+        const size_t csz = static_cast<size_t>(h->nbf) * ncol[i];
+        const double* cl = Cl[i];
+        const double* cr = Cr ? Cr[i] : nullptr;
+        double* gW = dW;
+        if (!device) {
+            double* raw = h->ws_craw.get<double>(2 * csz);
+            hcheck(hipMemcpyAsync(raw, cl, csz * sizeof(double), hipMemcpyHostToDevice, h->stream), "upload C_left");
+            cl = raw;
+            if (cr) {
+                hcheck(hipMemcpyAsync(raw + csz, cr, csz * sizeof(double), hipMemcpyHostToDevice, h->stream),
+                       "upload C_right");
+                cr = raw + csz;
+            }
+            gW = h->ws_K.get<double>(nn);
+            if (accumulate) hcheck(hipMemcpyAsync(gW, dW, nn * sizeof(double), hipMemcpyHostToDevice, h->stream), "up");
+        }
+        compute_wk_pair(h, cl, cr, ncol[i], gW, accumulate);
+        if (!device) {
+            hcheck(hipMemcpyAsync(dW, gW, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download wK");
+            hcheck(hipStreamSynchronize(h->stream), "compute wK");
+        }
+    }
+    finish_timers(h);
+}
+
 // Adjoint of E = tr(D(2J - K)), D = C C^T, at FP64 (the DF gradient terms of
 // scfgrad/jk_grad.cc:294-475 contract the same c_Q = d_Q and (Q|ij) = C^T B_Q C):
 //   dE/dC = 4 (2J - K) C,   Gamma_Qmn = dE/dB_Qmn = 4 d_Q D_mn - 2 (C (C^T X_Q) C^T)_mn, X_Q = B_Q C.
@@ -775,6 +1278,7 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
 // gam_dev (device, [q1-q0][nbf][nbf]) or, when gam_host is set, are copied to host memory.
 void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double* dEdC, double* gam_dev,
                   double* gam_host) {
+    if (h->ooc) fail(DFJK_ERR_STATE, "DFJK: the energy adjoint needs B in core (subalgorithm INCORE)");
     const hipStream_t s = h->stream;
     const int nbf = h->nbf, nbfp = h->nbfp;
     const size_t nn = static_cast<size_t>(nbf) * nbf;
@@ -920,8 +1424,15 @@ int dfjk_destroy(dfjk_handle* h) {
     return guarded([&] {
         if (!h) return;
         if (h->own_stream) (void)hipStreamSynchronize(h->own_stream);
-        hipStream_t s = h->own_stream;
+        if (h->cstream) (void)hipStreamSynchronize(h->cstream);
+        hipStream_t s = h->own_stream, cs = h->cstream;
+        for (int i = 0; i < 2; ++i) {
+            if (h->ev_copied[i]) (void)hipEventDestroy(h->ev_copied[i]);
+            if (h->ev_free[i]) (void)hipEventDestroy(h->ev_free[i]);
+        }
+        if (h->hB) (void)hipHostFree(h->hB);
         delete h;
+        if (cs) (void)hipStreamDestroy(cs);
         if (s) (void)hipStreamDestroy(s);
     });
 }
@@ -955,6 +1466,27 @@ int dfjk_set_do_K(dfjk_handle* h, int v) {
     return guarded([&] {
         require_state(h, false);
         h->do_K = v != 0;
+    });
+}
+int dfjk_set_do_wK(dfjk_handle* h, int v) {
+    return guarded([&] {
+        require_state(h, false);
+        h->do_wK = v != 0;
+    });
+}
+int dfjk_set_subalgorithm(dfjk_handle* h, int v) {
+    return guarded([&] {
+        require_state(h, false);
+        if (v < DFJK_SUBALGO_AUTO || v > DFJK_SUBALGO_OUT_OF_CORE) fail(DFJK_ERR_ARG, "unknown subalgorithm");
+        if (h->initialized) fail(DFJK_ERR_STATE, "the subalgorithm must be chosen before initialize()");
+        h->subalgo = v;
+    });
+}
+int dfjk_set_omega(dfjk_handle* h, double omega) {
+    return guarded([&] {
+        require_state(h, false);
+        if (!(omega >= 0.0) || !std::isfinite(omega)) fail(DFJK_ERR_ARG, "omega must be finite and >= 0");
+        h->omega = omega;  // a built attenuated tensor is rebuilt at the next wK compute
     });
 }
 int dfjk_set_omp_nthread(dfjk_handle* h, int n) {
@@ -1018,6 +1550,7 @@ int dfjk_memory_estimate(dfjk_handle* h, size_t* doubles) {
         }
         const size_t nbfp = static_cast<size_t>(round_up(h->nbf, TILE));
         size_t est = static_cast<size_t>(h->naux) * ntp * TILE2;   // resident B
+        if (h->do_wK) est *= 2;                                    // and the attenuated Bw
         est += static_cast<size_t>(h->naux) * h->naux;            // metric (fit only)
         est += 4 * static_cast<size_t>(h->nbf) * h->nbf;          // D, J, K, staging
         est += 2 * nbfp * 320;                                    // padded C panels
@@ -1031,6 +1564,8 @@ int dfjk_initialize(dfjk_handle* h) {
         if (h->initialized) return;
         ensure_stream(h);
         if (h->tensor_mode) {
+            if (h->subalgo == DFJK_SUBALGO_OUT_OF_CORE)
+                fail(DFJK_ERR_STATE, "DFJK: tensor-level handles are in core only");
             h->fun_mask.assign(static_cast<size_t>(h->nbf) * h->nbf, 1);
             h->nsig = static_cast<int64_t>(h->nbf) * h->nbf;
             build_tiles(h);
@@ -1045,34 +1580,82 @@ int dfjk_initialize(dfjk_handle* h) {
     });
 }
 
+namespace {
+// Pack a dense symmetric [naux][nbf][nbf] tensor (host or device) into the tiled layout of dst.
+void set_dense(dfjk_handle* h, const double* B, int on_device, DevBuf& dst) {
+    require_state(h, true);
+    if (h->ooc) fail(DFJK_ERR_STATE, "DFJK: dense tensors are set in core only");
+    if (!B) fail(DFJK_ERR_ARG, "null B");
+    const size_t nn = static_cast<size_t>(h->nbf) * h->nbf;
+    if (h->fun_mask.size() != nn) fail(DFJK_ERR_INTERNAL, "screening mask missing");
+    const TileGeom g = h->geom();
+    const hipStream_t s = h->stream;
+    double* out = dst.get<double>(b_doubles(h));
+    if (on_device) {
+        for (int q0 = 0; q0 < h->naux; q0 += 32768) {
+            const int nq = std::min(h->naux - q0, 32768);
+            launch_pack_dense(g, B + static_cast<size_t>(q0) * nn, h->d_mask.as<uint8_t>(), q0, nq, out, s);
+        }
+    } else {
+        // stage host Q-slices of about 256 MiB through a device buffer
+        const int qc = static_cast<int>(std::max<size_t>(1, std::min<size_t>(32768, (size_t(32) << 20) / nn)));
+        double* stage = h->ws_XL.get<double>(nn * qc);
+        for (int q0 = 0; q0 < h->naux; q0 += qc) {
+            const int nq = std::min(h->naux - q0, qc);
+            hcheck(hipMemcpyAsync(stage, B + static_cast<size_t>(q0) * nn, nn * nq * sizeof(double),
+                                  hipMemcpyHostToDevice, s),
+                   "upload B");
+            launch_pack_dense(g, stage, h->d_mask.as<uint8_t>(), q0, nq, out, s);
+        }
+    }
+    hcheck(hipStreamSynchronize(s), "set B");
+}
+
+// Unpack Q in [q0,q1) of a tiled tensor into dense [q1-q0][nbf][nbf] host memory.
+void get_dense(dfjk_handle* h, const DevBuf& src, int q0, int q1, double* out) {
+    require_state(h, true);
+    if (!out || q0 < 0 || q1 > h->naux || q0 >= q1) fail(DFJK_ERR_ARG, "bad range");
+    if (!src.p && !((&src == &h->dB) && h->ooc)) fail(DFJK_ERR_STATE, "tensor not built");
+    const int n = h->nbf;
+    const size_t nn = static_cast<size_t>(n) * n;
+    const size_t qs = static_cast<size_t>(h->ntp) * TILE2;
+    std::vector<double> packed(qs);
+    const bool host = (&src == &h->dB) && h->ooc;
+    for (int Q = q0; Q < q1; ++Q) {
+        if (host)
+            std::memcpy(packed.data(), h->hB + Q * qs, qs * sizeof(double));
+        else
+            hcheck(hipMemcpy(packed.data(), src.as<double>() + Q * qs, qs * sizeof(double), hipMemcpyDeviceToHost),
+                   "download B");
+        double* o = out + (Q - q0) * nn;
+        std::fill(o, o + nn, 0.0);
+        for (int t = 0; t < h->ntp; ++t) {
+            const int2 IJ = h->tiles[t];
+            for (int r = 0; r < TILE; ++r)
+                for (int c = 0; c < TILE; ++c) {
+                    const int m = IJ.x * TILE + r, q = IJ.y * TILE + c;
+                    if (m >= n || q >= n) continue;
+                    const double v = packed[static_cast<size_t>(t) * TILE2 + r * TILE + c];
+                    o[static_cast<size_t>(m) * n + q] = v;
+                    o[static_cast<size_t>(q) * n + m] = v;
+                }
+        }
+    }
+}
+}  // namespace
+
 int dfjk_set_B_dense(dfjk_handle* h, const double* B, int on_device) {
     return guarded([&] {
         require_state(h, true);
-        if (!B) fail(DFJK_ERR_ARG, "null B");
-        const size_t nn = static_cast<size_t>(h->nbf) * h->nbf;
-        if (h->fun_mask.size() != nn) fail(DFJK_ERR_INTERNAL, "screening mask missing");
         invalidate_copies(h);
-        const TileGeom g = h->geom();
-        const hipStream_t s = h->stream;
-        if (on_device) {
-            for (int q0 = 0; q0 < h->naux; q0 += 32768) {
-                const int nq = std::min(h->naux - q0, 32768);
-                launch_pack_dense(g, B + static_cast<size_t>(q0) * nn, h->d_mask.as<uint8_t>(), q0, nq,
-                                  h->dB.as<double>(), s);
-            }
-        } else {
-            // stage host Q-slices of about 256 MiB through a device buffer
-            const int qc = static_cast<int>(std::max<size_t>(1, std::min<size_t>(32768, (size_t(32) << 20) / nn)));
-            double* stage = h->ws_XL.get<double>(nn * qc);
-            for (int q0 = 0; q0 < h->naux; q0 += qc) {
-                const int nq = std::min(h->naux - q0, qc);
-                hcheck(hipMemcpyAsync(stage, B + static_cast<size_t>(q0) * nn, nn * nq * sizeof(double),
-                                      hipMemcpyHostToDevice, s),
-                       "upload B");
-                launch_pack_dense(g, stage, h->d_mask.as<uint8_t>(), q0, nq, h->dB.as<double>(), s);
-            }
-        }
-        hcheck(hipStreamSynchronize(s), "set B");
+        set_dense(h, B, on_device, h->dB);
+    });
+}
+
+int dfjk_set_Bw_dense(dfjk_handle* h, const double* Bw, int on_device) {
+    return guarded([&] {
+        set_dense(h, Bw, on_device, h->dBw);
+        h->bw_omega = h->omega;
     });
 }
 
@@ -1094,30 +1677,14 @@ int dfjk_fill_synthetic(dfjk_handle* h, uint64_t seed) {
 }
 
 int dfjk_get_B_dense(dfjk_handle* h, int q0, int q1, double* out) {
+    return guarded([&] { get_dense(h, h->dB, q0, q1, out); });
+}
+
+int dfjk_get_Bw_dense(dfjk_handle* h, int q0, int q1, double* out) {
     return guarded([&] {
         require_state(h, true);
-        if (!out || q0 < 0 || q1 > h->naux || q0 >= q1) fail(DFJK_ERR_ARG, "bad range");
-        const int n = h->nbf;
-        const size_t nn = static_cast<size_t>(n) * n;
-        const size_t qs = static_cast<size_t>(h->ntp) * TILE2;
-        std::vector<double> packed(qs);
-        for (int Q = q0; Q < q1; ++Q) {
-            hcheck(hipMemcpy(packed.data(), h->dB.as<double>() + Q * qs, qs * sizeof(double), hipMemcpyDeviceToHost),
-                   "download B");
-            double* o = out + (Q - q0) * nn;
-            std::fill(o, o + nn, 0.0);
-            for (int t = 0; t < h->ntp; ++t) {
-                const int2 IJ = h->tiles[t];
-                for (int r = 0; r < TILE; ++r)
-                    for (int c = 0; c < TILE; ++c) {
-                        const int m = IJ.x * TILE + r, q = IJ.y * TILE + c;
-                        if (m >= n || q >= n) continue;
-                        const double v = packed[static_cast<size_t>(t) * TILE2 + r * TILE + c];
-                        o[static_cast<size_t>(m) * n + q] = v;
-                        o[static_cast<size_t>(q) * n + m] = v;
-                    }
-            }
-        }
+        if (!h->tensor_mode && h->bw_omega != h->omega) build_Bw(h);
+        get_dense(h, h->dBw, q0, q1, out);
     });
 }
 
@@ -1129,6 +1696,16 @@ int dfjk_compute(dfjk_handle* h, int n, const double* const* Cl, const double* c
 int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
                         double* const* D, double* const* J, double* const* K, int accumulate) {
     return guarded([&] { compute_impl(h, n, Cl, Cr, ncol, D, J, K, accumulate, true); });
+}
+
+int dfjk_compute_wK(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                    double* const* wK, int accumulate) {
+    return guarded([&] { compute_wk_impl(h, n, Cl, Cr, ncol, wK, accumulate, false); });
+}
+
+int dfjk_compute_wK_device(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
+                           double* const* wK, int accumulate) {
+    return guarded([&] { compute_wk_impl(h, n, Cl, Cr, ncol, wK, accumulate, true); });
 }
 
 int dfjk_energy_adjoint(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB) {
@@ -1162,6 +1739,14 @@ int dfjk_get_info(dfjk_handle* h, int* nbf, int* naux, int* ntile_pairs, int64_t
     });
 }
 
+int dfjk_get_out_of_core(dfjk_handle* h, int* ooc) {
+    return guarded([&] {
+        require_state(h, false);
+        if (!ooc) fail(DFJK_ERR_ARG, "null output");
+        *ooc = h->ooc ? 1 : 0;
+    });
+}
+
 int dfjk_finalize(dfjk_handle* h) {
     return guarded([&] {
         require_state(h, false);
@@ -1169,7 +1754,8 @@ int dfjk_finalize(dfjk_handle* h) {
         DevBuf* ws[] = {&h->ws_craw, &h->ws_cl,  &h->ws_cr,  &h->ws_D,    &h->ws_W,  &h->ws_dpart, &h->ws_d,
                         &h->ws_Jpart, &h->ws_XL, &h->ws_XR,  &h->ws_P,    &h->ws_Pd, &h->ws_J,     &h->ws_K,
                         &h->ws_G,    &h->ws_sign, &h->ws_c32, &h->ws_cbf, &h->ws_cmap, &h->ws_ct,   &h->ws_red,
-                        &h->ws_Y,    &h->ws_PC,  &h->ws_gam, &h->ws_dc};
+                        &h->ws_Y,    &h->ws_PC,  &h->ws_gam, &h->ws_dc, &h->ws_stage[0], &h->ws_stage[1],
+                        &h->ws_stage32, &h->ws_stagebf};
         for (DevBuf* b : ws) b->release();
     });
 }

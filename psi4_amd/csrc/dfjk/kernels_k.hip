@@ -218,55 +218,43 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             Cs[buf][kk * CP + cc] = rc[it];
         }
     };
-    // fragments of a whole step (KS/4 sub-steps), read before the next step's loads are
-    // issued so their LDS latency overlaps the index lookups and address arithmetic
+    // One step's products: per 4-k sub-step the A fragments of the next sub-step and the C
+    // fragment of the next column block are read while the current block's MFMAs run, so
+    // the LDS reads overlap the MFMAs (consecutive updates of one accumulator are 4*NBW
+    // MFMAs apart).
     constexpr int NSUB = KS / 4;
-    // sub-steps whose fragments are read before the loads (the FP64 5-block shape has no
-    // register room for them next to the staging registers and reads them after)
-    constexpr int NPRE = (F64 && NBW == 5) ? 0 : NSUB;
-    T fa[NSUB][4], fb[NSUB][NBW];
-    auto frags = [&](int buf, int u0, int u1) {
-#pragma unroll
-        for (int u = u0; u < u1; ++u) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                fa[u][i] = As[buf][(wm * 64 + i * 16 + (lane & 15)) * AP + 4 * u + (lane >> 4)];
-#pragma unroll
-            for (int j = 0; j < NBW; ++j)
-                fb[u][j] = Cs[buf][(4 * u + (lane >> 4)) * CP + wn * NBW * 16 + j * 16 + (lane & 15)];
-        }
-    };
-    auto mma = [&]() {
-#pragma unroll
-        for (int u = 0; u < NSUB; ++u)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < NBW; ++j) {
-                    if constexpr (F64)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][i], fb[u][j], acc[i][j], 0, 0, 0);
-                    else
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[u][i], fb[u][j], acc[i][j], 0, 0, 0);
-                }
-    };
-    // per-sub-step reads and MFMAs, for the shape that reads its fragments after the loads
     auto compute = [&](int buf) {
-#pragma unroll
-        for (int ks = 0; ks < KS; ks += 4) {
-            T a[4], b[NBW];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[buf][(wm * 64 + i * 16 + (lane & 15)) * AP + ks + (lane >> 4)];
-#pragma unroll
-            for (int j = 0; j < NBW; ++j) b[j] = Cs[buf][(ks + (lane >> 4)) * CP + wn * NBW * 16 + j * 16 + (lane & 15)];
+        T a[2][4], b[2];
+        auto aread = [&](int u, int slot) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
+                a[slot][i] = As[buf][(wm * 64 + i * 16 + (lane & 15)) * AP + 4 * u + (lane >> 4)];
+        };
+        auto bread = [&](int u, int j, int slot) {
+            b[slot] = Cs[buf][(4 * u + (lane >> 4)) * CP + wn * NBW * 16 + j * 16 + (lane & 15)];
+        };
+        aread(0, 0);
+        bread(0, 0, 0);
 #pragma unroll
-                for (int j = 0; j < NBW; ++j) {
+        for (int u = 0; u < NSUB; ++u) {
+#pragma unroll
+            for (int j = 0; j < NBW; ++j) {
+                const int t = u * NBW + j;  // running block index: C fragment slot t & 1
+                if (j + 1 < NBW)
+                    bread(u, j + 1, (t + 1) & 1);
+                else if (u + 1 < NSUB)
+                    bread(u + 1, 0, (t + 1) & 1);
+                if (j == 0 && u + 1 < NSUB) aread(u + 1, (u + 1) & 1);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
                     if constexpr (F64)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u & 1][i], b[t & 1], acc[i][j], 0, 0, 0);
                     else
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u & 1][i], b[t & 1], acc[i][j], 0, 0, 0);
                 }
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     };
 
@@ -278,15 +266,9 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         int buf = 0;
         for (int st = 0; st < nsteps; ++st) {
             const bool more = st + 1 < nsteps;
-            if constexpr (NPRE > 0) {
-                frags(buf, 0, NPRE);
-                if (more) gload(st + 1);
-                if (NPRE < NSUB) frags(buf, NPRE, NSUB);
-                mma();
-            } else {
-                if (more) gload(st + 1);
-                compute(buf);
-            }
+            if (more) gload(st + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(buf);
             if (more) lstore(buf ^ 1, st + 1);
             __syncthreads();
             buf ^= 1;
@@ -305,6 +287,11 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
                 Xq[static_cast<size_t>(m) * ldc + col0 + wn * NBW * 16 + j * 16 + (lane & 15)] = acc[i][j][r];
             }
 }
+
+// BF16 LDS tiles of 32 k per row (64 bytes): the four 16-byte chunks of row r are stored at
+// chunk c ^ ((r>>1)&3), so fragment reads (16 rows x one chunk per 16 lanes) and row-segment
+// stores (4 lanes per row) are free of bank conflicts.
+__device__ inline int kg_swz(int row, int chunk) { return row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3); }
 
 // =====================================================================================
 // Half-transform, 3-term BF16 split: X = Bh Ch + Bh Cl + Bl Ch with FP32 accumulation.
@@ -325,15 +312,19 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     constexpr int RT = 2 * WM;
     constexpr int ROWS = 64 * WM;
     constexpr int PAN = 64 * NBW;
-    constexpr int KP = 40;                   // LDS pitch in bf16 (80 B rows, 16 B aligned)
-    constexpr int A_SZ = ROWS * KP;
-    constexpr int C_SZ = PAN * KP;
+    constexpr int A_SZ = ROWS * 32;          // swizzled 64-byte rows (kg_swz)
+    constexpr int C_SZ = PAN * 32;
     constexpr int NAV = ROWS * 4 / NTHR;     // 16-byte A vectors per thread per array (= 1)
     constexpr int NCV = PAN * 4 / NTHR;      // full rounds of 16-byte C vectors per thread per array
     constexpr int NCV_R = (PAN * 4) % NTHR;  // vectors in the last partial round
     static_assert(NAV == 1, "one A vector per thread");
     static_assert(NCV + (NCV_R ? 1 : 0) <= 5, "C staging slots");
-    __shared__ uint16_t Ls[2][2 * A_SZ + 2 * C_SZ];
+    static_assert(NCV_R % 4 == 0, "whole C rows per partial round");
+    constexpr int NSLOT = NCV + (NCV_R ? 1 : 0);  // C vector rounds per thread
+    // lanes with no C vector in the partial round store theirs into a slot of their own
+    constexpr int NDUM = NCV_R ? NTHR - NCV_R : 1;
+    __shared__ __align__(16) uint16_t Ls[2][2 * A_SZ + 2 * C_SZ];
+    __shared__ __align__(16) uint16_t Ldum[2][NDUM * 8];
     __shared__ StepList<RT> sl;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -391,59 +382,62 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             h = *reinterpret_cast<const v4u*>(CTh + co);
             l = *reinterpret_cast<const v4u*>(CTl + co);
         };
-        cload(0, c0h, c0l);
-        if (NCV >= 1) cload(1, c1h, c1l);
-        if (NCV >= 2) cload(2, c2h, c2l);
-        if (NCV >= 3) cload(3, c3h, c3l);
-        if (NCV >= 4) cload(4, c4h, c4l);
+        if (NSLOT > 0) cload(0, c0h, c0l);
+        if (NSLOT > 1) cload(1, c1h, c1l);
+        if (NSLOT > 2) cload(2, c2h, c2l);
+        if (NSLOT > 3) cload(3, c3h, c3l);
+        if (NSLOT > 4) cload(4, c4h, c4l);
     };
     auto lstore = [&](int buf, const v4u& ah, const v4u& al) {
         uint16_t* Ah = Ls[buf];
         uint16_t* Al = Ah + A_SZ;
         uint16_t* Ch = Al + A_SZ;
         uint16_t* Cl = Ch + C_SZ;
-        const int r = f >> 2, c8 = 8 * (f & 3);
-        *reinterpret_cast<v4u*>(Ah + (sub * 32 + r) * KP + c8) = ah;
-        *reinterpret_cast<v4u*>(Al + (sub * 32 + r) * KP + c8) = al;
-        // lanes with no vector in the partial round store theirs into the unused pitch padding
-        // (k 32..39) of some column, so every loaded value has an unconditional use and the
-        // loads stay where they are issued
+        const int ao = kg_swz(sub * 32 + (f >> 2), f & 3);
+        *reinterpret_cast<v4u*>(Ah + ao) = ah;
+        *reinterpret_cast<v4u*>(Al + ao) = al;
+        // every loaded value has an unconditional use, so the loads stay where they are issued
         auto cstore = [&](int it, const v4u& h, const v4u& l) {
             const int v = tid + NTHR * it;
-            int o;
-            if (it < NCV || v < NCV_R + NTHR * NCV)
-                o = (v >> 2) * KP + 8 * (v & 3);
-            else
-                o = ((v - PAN * 4) % PAN) * KP + 32;
-            *reinterpret_cast<v4u*>(Ch + o) = h;
-            *reinterpret_cast<v4u*>(Cl + o) = l;
+            uint16_t *dh, *dl;
+            if (v < PAN * 4) {
+                const int o = kg_swz(v >> 2, v & 3);
+                dh = Ch + o;
+                dl = Cl + o;
+            } else {
+                dh = &Ldum[0][(v - PAN * 4) * 8];
+                dl = &Ldum[1][(v - PAN * 4) * 8];
+            }
+            *reinterpret_cast<v4u*>(dh) = h;
+            *reinterpret_cast<v4u*>(dl) = l;
         };
-        cstore(0, c0h, c0l);
-        if (NCV >= 1) cstore(1, c1h, c1l);
-        if (NCV >= 2) cstore(2, c2h, c2l);
-        if (NCV >= 3) cstore(3, c3h, c3l);
-        if (NCV >= 4) cstore(4, c4h, c4l);
+        if (NSLOT > 0) cstore(0, c0h, c0l);
+        if (NSLOT > 1) cstore(1, c1h, c1l);
+        if (NSLOT > 2) cstore(2, c2h, c2l);
+        if (NSLOT > 3) cstore(3, c3h, c3l);
+        if (NSLOT > 4) cstore(4, c4h, c4l);
     };
-    auto compute = [&](int buf) {
+    // One step's products: all fragments first (one LDS wait), then the MFMAs product by
+    // product (consecutive updates of one accumulator are 4*NBW MFMAs apart), then the next
+    // step's staging into the other buffer.
+    auto compute = [&](int buf, auto&& store_next) {
         const uint16_t* Ah = Ls[buf];
         const uint16_t* Al = Ah + A_SZ;
         const uint16_t* Ch = Al + A_SZ;
         const uint16_t* Cl = Ch + C_SZ;
-        const int k8 = 8 * (lane >> 4);
-        // hi fragments and C-lo first; A-lo is read after the hi*lo products are issued, into
-        // the registers C-lo frees, so consecutive updates of one accumulator stay 4*NBW
-        // MFMAs apart with one fragment set fewer live
-        bf8_t ah[4], bh[NBW], bl[NBW];
+        const int fc = lane >> 4;
+        bf8_t ah[4], al[4], bh[NBW], bl[NBW];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int row = wm * 64 + i * 16 + (lane & 15);
-            ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ah + row * KP + k8));
+            const int o = kg_swz(wm * 64 + i * 16 + (lane & 15), fc);
+            ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ah + o));
+            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + o));
         }
 #pragma unroll
         for (int j = 0; j < NBW; ++j) {
-            const int col = wn * NBW * 16 + j * 16 + (lane & 15);
-            bh[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ch + col * KP + k8));
-            bl[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Cl + col * KP + k8));
+            const int o = kg_swz(wn * NBW * 16 + j * 16 + (lane & 15), fc);
+            bh[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ch + o));
+            bl[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Cl + o));
         }
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
@@ -453,16 +447,12 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        bf8_t al[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = wm * 64 + i * 16 + (lane & 15);
-            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + row * KP + k8));
-        }
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        store_next();
     };
     // Steps run in pairs so the two A sets are named statically; the loop body has no
     // branches around its loads and stores (loads past the last step re-read it, and their
@@ -479,19 +469,15 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             gloadC(st + 1);
             gloadA(st + 2, a0h, a0l);
             __builtin_amdgcn_sched_barrier(0);
-            compute(0);
-            __builtin_amdgcn_sched_barrier(0);
-            lstore(1, a1h, a1l);
+            compute(0, [&] { lstore(1, a1h, a1l); });
             __syncthreads();
             gloadC(st + 2);
             gloadA(st + 3, a1h, a1l);
             __builtin_amdgcn_sched_barrier(0);
-            compute(1);
-            __builtin_amdgcn_sched_barrier(0);
-            lstore(0, a0h, a0l);
+            compute(1, [&] { lstore(0, a0h, a0l); });
             __syncthreads();
         }
-        if (st < nsteps) compute(0);  // odd step count: the last step is in buffer 0
+        if (st < nsteps) compute(0, [] {});  // odd step count: the last step is in buffer 0
     }
     const size_t xq = static_cast<size_t>(q) * xrows * ldc;
 #pragma unroll
@@ -660,7 +646,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // (FP64 totals would not fit beside the fragments in 256 registers): a sum of N terms then
 // rounds like two chains of about sqrt(N) additions instead of one chain of N.
 // =====================================================================================
-__device__ inline int kg_swz(int row, int chunk) { return row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3); }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_bf3(
     const uint16_t* __restrict__ Ah_, const uint16_t* __restrict__ Al_, const uint16_t* __restrict__ Bh_,

@@ -244,9 +244,11 @@ Shell dummy_s(const Shell& like) {
     return s;
 }
 
-// Per-thread scratch.
+// Per-thread scratch.  omega > 0 selects the attenuated operator erf(omega r)/r for the
+// two-electron integrals (0: plain Coulomb).
 struct Work {
     std::vector<double> tmp, R, W, cart, half;
+    double omega = 0.0;
 };
 
 // R^0_{tuv}(alpha, PQ) for t+u+v <= L into R (size nherm(L)).
@@ -309,6 +311,15 @@ void eri_cart(const PairData& bra, const PairData& ket, double* out, Work& w) {
             double p = pb.p, q = pk.p;
             double alpha = p * q / (p + q);
             double pref = 2.0 * std::pow(kPi, 2.5) / (p * q * std::sqrt(p + q)) * pb.pref * pk.pref;
+            if (w.omega > 0.0) {
+                // erf(omega r)/r acts like 1/r between the two distributions with one of them
+                // smeared by a Gaussian of exponent omega^2: 1/alpha' = 1/alpha + 1/omega^2, and
+                // the prefactor scales by sqrt(alpha'/alpha)
+                const double w2 = w.omega * w.omega;
+                const double aw = alpha * w2 / (alpha + w2);
+                pref *= std::sqrt(aw / alpha);
+                alpha = aw;
+            }
             double PQ[3] = {pb.P[0] - pk.P[0], pb.P[1] - pk.P[1], pb.P[2] - pk.P[2]};
             hermite_R(L, alpha, PQ, w);
             const double* R = w.R.data();
@@ -448,6 +459,14 @@ void normalize_shell(Shell& s) {
 const std::vector<double>& cart2sph(int l) {
     static SphTable tab;
     return tab.c2s[l];
+}
+
+const double* boys_grid(int* nt, int* ncol, double* h, double* tmax) {
+    *nt = kBoysNT;
+    *ncol = kBoysNCol;
+    *h = kBoysH;
+    *tmax = kBoysTMax;
+    return boys_table().t.data();
 }
 
 void boys(int nmax, double T, double* F) {
@@ -798,7 +817,7 @@ void potential(const Basis& b, int natom, const double* Z, const double* xyz, do
     one_electron(b, OneE::V, natom, Z, xyz, V);
 }
 
-void metric(const Basis& aux, double* J, int nthreads) {
+void metric(const Basis& aux, double* J, int nthreads, double omega) {
     const int ns = static_cast<int>(aux.shells.size());
     std::vector<PairData> pd(ns);
     std::vector<Shell> dum(ns);
@@ -810,6 +829,7 @@ void metric(const Basis& aux, double* J, int nthreads) {
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
     for (int P = 0; P < ns; ++P) {
         Work w;
+        w.omega = omega;
         for (int Q = 0; Q <= P; ++Q) {
             const Shell& sp = aux.shells[P];
             const Shell& sq = aux.shells[Q];
@@ -829,7 +849,7 @@ void metric(const Basis& aux, double* J, int nthreads) {
 }
 
 void three_index(const Basis& prim, const Basis& aux, const int64_t* dst, int64_t p_stride, const uint8_t* shell_mask,
-                 double* out, int nthreads) {
+                 double* out, int nthreads, double omega) {
     const int na = static_cast<int>(aux.shells.size());
     const int np = static_cast<int>(prim.shells.size());
     const int nbf = prim.nbf;
@@ -847,6 +867,7 @@ void three_index(const Basis& prim, const Basis& aux, const int64_t* dst, int64_
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
     for (int ip = 0; ip < npairs; ++ip) {
         Work w;
+        w.omega = omega;
         const int M = pairs[ip].first, N = pairs[ip].second;
         const Shell& sm = prim.shells[M];
         const Shell& sn = prim.shells[N];
@@ -1038,7 +1059,7 @@ double schwarz_diagonal(const Basis& prim, double* fun_max, double* shell_max, i
     return max_val;
 }
 
-void four_index(const Basis& b, double* out, int nthreads) {
+void four_index(const Basis& b, double* out, int nthreads, double omega) {
     const int ns = static_cast<int>(b.shells.size());
     const size_t n = b.nbf;
     std::vector<std::pair<int, int>> pairs;
@@ -1050,6 +1071,7 @@ void four_index(const Basis& b, double* out, int nthreads) {
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads)
     for (int i = 0; i < np; ++i) {
         Work w;
+        w.omega = omega;
         for (int j = 0; j < np; ++j) {
             const Shell* s4[4] = {&b.shells[pairs[i].first], &b.shells[pairs[i].second], &b.shells[pairs[j].first],
                                   &b.shells[pairs[j].second]};

@@ -49,6 +49,9 @@ const std::vector<double>& cart2sph(int l);
 
 // Boys function F_n(T) for n = 0..nmax written into F[0..nmax].
 void boys(int nmax, double T, double* F);
+// The tabulated grid behind boys(): F_n(i h) at [i * ncol + n] for i < nt; Taylor expansion
+// of order 6 about the nearest grid point below tmax, asymptotic form above.
+const double* boys_grid(int* nt, int* ncol, double* h, double* tmax);
 
 // ---- integral drivers (all outputs row-major, FP64) ----
 void overlap(const Basis& b, double* S);
@@ -58,12 +61,14 @@ void dipole(const Basis& b, const double* origin, double* M);
 // <mu| (r - origin) x nabla |nu> for x, y, z into M[3][nbf][nbf] (antisymmetric; no factor -i).
 void angular_momentum(const Basis& b, const double* origin, double* M);
 void potential(const Basis& b, int natom, const double* Z, const double* xyz, double* V);
+// Two-electron drivers take omega: 0 for the Coulomb operator 1/r, > 0 for the attenuated
+// erf(omega r)/r of range-separated exchange (psi4's erf_eri, libmints/integral.h).
 // (P|Q), naux x naux.
-void metric(const Basis& aux, double* J, int nthreads);
+void metric(const Basis& aux, double* J, int nthreads, double omega = 0.0);
 // (P|mn): out[P * p_stride + dst[m * nbf + n]] for every (m,n) whose dst >= 0.
 // Shell pairs (MU,NU) with shell_mask[MU*nshell+NU] == 0 are skipped (mask may be null).
 void three_index(const Basis& prim, const Basis& aux, const int64_t* dst, int64_t p_stride,
-                 const uint8_t* shell_mask, double* out, int nthreads);
+                 const uint8_t* shell_mask, double* out, int nthreads, double omega = 0.0);
 // Schwarz diagonal (mn|mn) for all function pairs: fun_max nbf*nbf (both triangles),
 // shell_max nshell*nshell, following dfhelper.cc:322-366. Returns max value.
 double schwarz_diagonal(const Basis& prim, double* fun_max, double* shell_max, int nthreads);
@@ -80,6 +85,6 @@ void three_index_grad(const Basis& prim, const Basis& aux, const double* G, doub
 // sum_{P,Q} W[P][Q] d(P|Q)/dR, W symmetric: into g_aux[naux_shell][3].
 void metric_grad(const Basis& aux, const double* W, double* g_aux, int nthreads);
 // Full (mn|ls) over a basis, nbf^4, for brute-force checks of tiny systems.
-void four_index(const Basis& b, double* out, int nthreads);
+void four_index(const Basis& b, double* out, int nthreads, double omega = 0.0);
 
 }  // namespace p4a

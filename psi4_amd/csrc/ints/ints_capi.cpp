@@ -91,6 +91,29 @@ int p4a_ints_three_index(const p4a_basis* prim, const p4a_basis* aux, const int6
     });
 }
 
+int p4a_ints_metric_erf(const p4a_basis* aux, double omega, double* J, int nthreads) {
+    return guarded([&] {
+        if (!(omega > 0.0)) throw std::runtime_error("omega must be positive");
+        p4a::metric(p4a::basis_from_desc(aux), J, nthreads, omega);
+    });
+}
+
+int p4a_ints_three_index_erf(const p4a_basis* prim, const p4a_basis* aux, double omega, const int64_t* dst,
+                             int64_t p_stride, const uint8_t* shell_mask, double* out, int nthreads) {
+    return guarded([&] {
+        if (!(omega > 0.0)) throw std::runtime_error("omega must be positive");
+        p4a::three_index(p4a::basis_from_desc(prim), p4a::basis_from_desc(aux), dst, p_stride, shell_mask, out,
+                         nthreads, omega);
+    });
+}
+
+int p4a_ints_four_index_erf(const p4a_basis* b, double omega, double* out, int nthreads) {
+    return guarded([&] {
+        if (!(omega > 0.0)) throw std::runtime_error("omega must be positive");
+        p4a::four_index(p4a::basis_from_desc(b), out, nthreads, omega);
+    });
+}
+
 int p4a_ints_schwarz(const p4a_basis* prim, double* fun_max, double* shell_max, double* max_val, int nthreads) {
     return guarded(
         [&] { *max_val = p4a::schwarz_diagonal(p4a::basis_from_desc(prim), fun_max, shell_max, nthreads); });

@@ -83,7 +83,6 @@ class JK {
         return d;
     }
     void initialize() {
-        if (do_wK_) throw std::runtime_error("MemDFJK on MI355X: wK (range-separated exchange) is not supported");
         if (wcombine_) throw std::runtime_error("MemDFJK on MI355X: wcombine is not supported");
         check(dfjk_initialize(h_));
         initialized_ = true;
@@ -94,8 +93,21 @@ class JK {
     void set_omp_nthread(int n) { check(dfjk_set_omp_nthread(h_, n)); nthread_ = n; }
     void set_do_J(bool v) { check(dfjk_set_do_J(h_, v)); do_J_ = v; }
     void set_do_K(bool v) { check(dfjk_set_do_K(h_, v)); do_K_ = v; }
-    void set_do_wK(bool v) { do_wK_ = v; }
-    void set_omega(double w) { omega_ = w; }
+    void set_do_wK(bool v) { check(dfjk_set_do_wK(h_, v)); do_wK_ = v; }
+    // SCF_SUBTYPE: AUTO, INCORE, OUT_OF_CORE (jk.h:1039 set_subalgo)
+    void set_subalgo(const std::string& v) {
+        int a;
+        if (v == "AUTO")
+            a = DFJK_SUBALGO_AUTO;
+        else if (v == "INCORE")
+            a = DFJK_SUBALGO_INCORE;
+        else if (v == "OUT_OF_CORE")
+            a = DFJK_SUBALGO_OUT_OF_CORE;
+        else
+            throw std::runtime_error("JK: SCF_SUBTYPE must be AUTO, INCORE or OUT_OF_CORE");
+        check(dfjk_set_subalgorithm(h_, a));
+    }
+    void set_omega(double w) { check(dfjk_set_omega(h_, w)); omega_ = w; }
     double get_omega() const { return omega_; }
     void set_wcombine(bool w) { wcombine_ = w; }
     bool get_wcombine() const { return wcombine_; }
@@ -149,16 +161,21 @@ class JK {
         }
         if (!initialized_) throw std::runtime_error("JK: initialize() must be called before compute()");
         // outputs are (re)allocated unless accumulating into existing ones of the right count
-        if (!accumulate_ || J_.size() != n) {
+        if (!accumulate_ || J_.size() != n || (do_wK_ && wK_.size() != n)) {
             D_.clear();
             J_.clear();
             K_.clear();
+            wK_.clear();
             for (size_t i = 0; i < n; ++i) {
                 D_.push_back(Arr({nbf_, nbf_}));
                 J_.push_back(Arr({nbf_, nbf_}));
                 K_.push_back(Arr({nbf_, nbf_}));
                 std::fill(J_.back().mutable_data(), J_.back().mutable_data() + nbf_ * nbf_, 0.0);
                 std::fill(K_.back().mutable_data(), K_.back().mutable_data() + nbf_ * nbf_, 0.0);
+                if (do_wK_) {
+                    wK_.push_back(Arr({nbf_, nbf_}));
+                    std::fill(wK_.back().mutable_data(), wK_.back().mutable_data() + nbf_ * nbf_, 0.0);
+                }
             }
         }
         std::vector<const double*> pl(n), pr(n);
@@ -175,6 +192,12 @@ class JK {
             check(dfjk_compute(h_, static_cast<int>(n), pl.data(), lr_symmetric ? nullptr : pr.data(), ncol.data(),
                                pd.data(), do_J_ ? pj.data() : nullptr, do_K_ ? pk.data() : nullptr,
                                accumulate_ ? 1 : 0));
+            if (do_wK_) {
+                std::vector<double*> pw(n);
+                for (size_t i = 0; i < n; ++i) pw[i] = wK_[i].mutable_data();
+                check(dfjk_compute_wK(h_, static_cast<int>(n), pl.data(), lr_symmetric ? nullptr : pr.data(),
+                                      ncol.data(), pw.data(), accumulate_ ? 1 : 0));
+            }
         }
     }
 
@@ -263,13 +286,32 @@ class JK {
         d["naux"] = naux;
         d["ntile_pairs"] = ntp;
         d["nsig_pairs"] = nsig;
+        int ooc = 0;
+        check(dfjk_get_out_of_core(h_, &ooc));
+        d["out_of_core"] = ooc != 0;
         return d;
     }
 
     py::list J() const { return to_list(J_); }
     py::list K() const { return to_list(K_); }
     py::list D() const { return to_list(D_); }
-    py::list wK() const { return py::list(); }
+    py::list wK() const { return to_list(wK_); }
+    void set_Bw_dense(Arr Bw) {
+        const int naux = info_naux();
+        if (Bw.ndim() != 3 || Bw.shape(0) != naux || Bw.shape(1) != nbf_ || Bw.shape(2) != nbf_)
+            throw std::runtime_error("JK: Bw must be naux x nbf x nbf");
+        check(dfjk_set_Bw_dense(h_, Bw.data(), 0));
+    }
+    Arr get_Bw_dense(int q0, int q1) {
+        Arr out({static_cast<py::ssize_t>(q1 - q0), static_cast<py::ssize_t>(nbf_), static_cast<py::ssize_t>(nbf_)});
+        check(dfjk_get_Bw_dense(h_, q0, q1, out.mutable_data()));
+        return out;
+    }
+    int info_naux() {
+        int naux = 0;
+        check(dfjk_get_info(h_, nullptr, &naux, nullptr, nullptr));
+        return naux;
+    }
     void finalize() { check(dfjk_finalize(h_)); }
 
     // MemDFJK::print_header (libfock/MemDFJK.cc:113-132)
@@ -288,7 +330,9 @@ class JK {
         os << buf;
         std::snprintf(buf, sizeof buf, "    Memory [MiB]:       %11ld\n", static_cast<long>((memory_ * 8L) / (1024L * 1024L)));
         os << buf;
-        std::snprintf(buf, sizeof buf, "    Algorithm:          %11s\n", "Device");
+        int ooc = 0;
+        if (initialized_) check(dfjk_get_out_of_core(h_, &ooc));
+        std::snprintf(buf, sizeof buf, "    Algorithm:          %11s\n", ooc ? "Host-stream" : "Device");
         os << buf;
         std::snprintf(buf, sizeof buf, "    Schwarz Cutoff:     %11.0E\n", cutoff_);
         os << buf;
@@ -313,7 +357,7 @@ class JK {
     int aux_q0_ = 0, aux_q1_ = 0;
     std::string name_;
     py::list Cl_, Cr_;
-    std::vector<Arr> D_, J_, K_;
+    std::vector<Arr> D_, J_, K_, wK_;
     bool initialized_ = false, do_J_ = true, do_K_ = true, do_wK_ = false, wcombine_ = false, accumulate_ = false;
     int print_ = 1, nthread_ = 1, precision_ = 0;
     size_t memory_ = 256000000;
@@ -343,6 +387,7 @@ PYBIND11_MODULE(_core, m) {
         .def("set_do_J", &JK::set_do_J)
         .def("set_do_K", &JK::set_do_K)
         .def("set_do_wK", &JK::set_do_wK)
+        .def("set_subalgo", &JK::set_subalgo)
         .def("set_omega", &JK::set_omega, py::arg("omega"))
         .def("get_omega", &JK::get_omega)
         .def("set_wcombine", &JK::set_wcombine, py::arg("wcombine"))
@@ -379,6 +424,8 @@ PYBIND11_MODULE(_core, m) {
              py::arg("dEdB"))
         .def("fill_synthetic", &JK::fill_synthetic)
         .def("get_B_dense", &JK::get_B_dense)
+        .def("set_Bw_dense", &JK::set_Bw_dense)
+        .def("get_Bw_dense", &JK::get_Bw_dense)
         .def("timings", &JK::timings)
         .def("info", &JK::info)
         .def("print_header", &JK::print_header);

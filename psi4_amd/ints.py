@@ -75,25 +75,35 @@ def potential(basis, mol):
     return V
 
 
-def metric(aux, nthreads=None):
+def metric(aux, nthreads=None, omega=None):
+    """(P|Q); with omega > 0 over erf(omega r)/r."""
     lib = load_ints_lib()
     n = aux.nbf()
     J = np.zeros((n, n))
     cb, _keep = aux.as_c()
-    _check(lib, lib.p4a_ints_metric(ctypes.byref(cb), _ptr(J), nthreads or default_threads()))
+    if omega:
+        _check(lib, lib.p4a_ints_metric_erf(ctypes.byref(cb), ctypes.c_double(omega), _ptr(J),
+                                            nthreads or default_threads()))
+    else:
+        _check(lib, lib.p4a_ints_metric(ctypes.byref(cb), _ptr(J), nthreads or default_threads()))
     return J
 
 
-def three_index(prim, aux, nthreads=None):
-    """Dense (P|mn) as an array [naux, nbf, nbf]."""
+def three_index(prim, aux, nthreads=None, omega=None):
+    """Dense (P|mn) as an array [naux, nbf, nbf]; with omega > 0 over erf(omega r)/r."""
     lib = load_ints_lib()
     n, na = prim.nbf(), aux.nbf()
     out = np.zeros((na, n, n))
     dst = np.arange(n * n, dtype=np.int64)
     cp, _k1 = prim.as_c()
     ca, _k2 = aux.as_c()
-    _check(lib, lib.p4a_ints_three_index(ctypes.byref(cp), ctypes.byref(ca), dst.ctypes.data_as(
-        ctypes.POINTER(ctypes.c_int64)), ctypes.c_int64(n * n), None, _ptr(out), nthreads or default_threads()))
+    dptr = dst.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    if omega:
+        _check(lib, lib.p4a_ints_three_index_erf(ctypes.byref(cp), ctypes.byref(ca), ctypes.c_double(omega), dptr,
+                                                 ctypes.c_int64(n * n), None, _ptr(out), nthreads or default_threads()))
+    else:
+        _check(lib, lib.p4a_ints_three_index(ctypes.byref(cp), ctypes.byref(ca), dptr, ctypes.c_int64(n * n), None,
+                                             _ptr(out), nthreads or default_threads()))
     return out
 
 
@@ -110,12 +120,17 @@ def schwarz(prim, nthreads=None):
     return fm, sm, mx.value
 
 
-def four_index(basis, nthreads=None):
+def four_index(basis, nthreads=None, omega=None):
+    """Full (mn|ls); with omega > 0 over erf(omega r)/r."""
     lib = load_ints_lib()
     n = basis.nbf()
     out = np.zeros((n, n, n, n))
     cb, _k = basis.as_c()
-    _check(lib, lib.p4a_ints_four_index(ctypes.byref(cb), _ptr(out), nthreads or default_threads()))
+    if omega:
+        _check(lib, lib.p4a_ints_four_index_erf(ctypes.byref(cb), ctypes.c_double(omega), _ptr(out),
+                                                nthreads or default_threads()))
+    else:
+        _check(lib, lib.p4a_ints_four_index(ctypes.byref(cb), _ptr(out), nthreads or default_threads()))
     return out
 
 

@@ -67,9 +67,15 @@ def test_jk_build_host_logic():
         JK.build(bs, screening="DENSITY")
     with pytest.raises(RuntimeError, match="only MEM_DF"):
         JK.build(bs, jk_type="DIRECT")
-    jk.set_do_wK(True)
-    with pytest.raises(RuntimeError, match="wK"):
+    est = jk.memory_estimate()
+    jk.set_do_wK(True)                  # the attenuated tensor doubles the resident part
+    assert jk.memory_estimate() - est == 464 * jk.info()["ntile_pairs"] * 1024
+    with pytest.raises(RuntimeError, match="omega"):
+        jk.set_omega(-1.0)
+    jk.set_wcombine(True)
+    with pytest.raises(RuntimeError, match="wcombine"):
         jk.initialize()
+    jk.set_wcombine(False)
     jk.set_do_wK(False)
     with pytest.raises(RuntimeError, match="initialize"):
         jk.C_left_add(np.zeros((96, 5)))
