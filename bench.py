@@ -57,7 +57,9 @@ def parse():
     ap.add_argument("--scf-cluster", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
                     help="instead of the synthetic build: RHF on a seeded water cluster of NX*NY*NZ molecules, "
                          "plain FP64 vs incremental mixed precision, then the energy adjoint")
-    ap.add_argument("--cpu-check", action="store_true", help="with --scf-cluster: also run the CPU oracle RHF")
+    ap.add_argument("--cpu-check", action="store_true",
+                    help="with --scf-cluster: the CPU oracle RHF (nbf <= 400), or the oracle J/K and energy at the "
+                         "converged orbitals on the device's tensor (larger clusters)")
     return ap.parse_args()
 
 
@@ -71,7 +73,7 @@ def scf_workload(args):
     nx, ny, nz = args.scf_cluster
     mol = water_cluster((nx, ny, nz), seed=0)
     prec = args.precision if args.precision != "fp64" else "bf16x3"
-    log(f"(H2O)_{nx * ny * nz}: setup (host integrals, device fit)")
+    log(f"(H2O)_{nx * ny * nz}: setup (integrals, device fit)")
     t0 = time.perf_counter()
     plain = RHF(mol)
     jk = plain.jk()
@@ -114,7 +116,39 @@ def scf_workload(args):
            "dE_mixed_vs_plain": abs(rm.energy - rp.energy),
            "adjoint": {"wall_s": round(t_adj, 3), "E_2": E2, "rel_<B,dE/dB>/2E-1": euler_B,
                        "rel_<C,dE/dC>/4E-1": euler_C}}
-    if args.cpu_check:
+    if args.cpu_check and nbf > 400:
+        # large clusters: the oracle's J/K contractions (compute_J_symm, compute_K) on the
+        # device's fitted tensor, Q-slice by Q-slice on the host cores, at each run's
+        # converged orbitals; E_cpu = E_nuc + 2<D,H> + <D, 2J - K>
+        from oracle.dfhelper import compute_J_symm, compute_K, prepare_sparsity
+        from psi4_amd import ints
+        H = ints.kinetic(plain.basis) + ints.potential(plain.basis, mol)
+        enuc = mol.nuclear_repulsion_energy()
+
+        def cpu_energy(Cfull, nocc):
+            Co = np.ascontiguousarray(Cfull[:, :nocc])
+            Dh = Co @ Co.T
+            Jc = np.zeros((nbf, nbf))
+            Kc = np.zeros((nbf, nbf))
+            for qa in range(0, naux, 200):
+                qb = min(naux, qa + 200)
+                Bq = jk.get_B_dense(qa, qb)
+                sp = prepare_sparsity(np.ones((nbf, nbf)), np.ones((1, 1)), 1.0, 1.0, qb - qa)
+                Ppq = np.ascontiguousarray(Bq.transpose(1, 0, 2)).reshape(-1)
+                del Bq
+                compute_J_symm(Ppq, sp, Dh, Jc)
+                compute_K(Ppq, sp, Co, Co, Kc, True)
+                if qa % 1000 == 0:
+                    log(f"  CPU oracle J/K: aux functions {qa}..{qb} of {naux}")
+            return enuc + 2.0 * np.vdot(Dh, H) + np.vdot(Dh, 2.0 * Jc - Kc)
+        log("CPU oracle J/K at the converged orbitals (device B, host contractions)")
+        t0 = time.perf_counter()
+        Ep = cpu_energy(rp.C, plain.nocc)
+        Em = cpu_energy(rm.C, mixed.nocc)
+        out["cpu_oracle"] = {"kind": "oracle compute_J_symm + compute_K on the device's fitted B at the converged C",
+                             "E_at_plain_C": Ep, "E_at_mixed_C": Em, "wall_s": round(time.perf_counter() - t0, 1),
+                             "dE_plain_vs_cpu": abs(rp.energy - Ep), "dE_mixed_vs_cpu": abs(rm.energy - Em)}
+    elif args.cpu_check:
         from oracle.dfhelper import MemDFJKOracle
         from oracle.rhf import rhf
         from psi4_amd import ints

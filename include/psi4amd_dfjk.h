@@ -58,7 +58,7 @@ enum dfjk_fitting {
 };
 
 enum dfjk_subalgorithm {      /* psi4 SCF_SUBTYPE (DFHelper subalgo_, dfhelper.h:370-376; jk.h:1039 set_subalgo) */
-    DFJK_SUBALGO_AUTO = 0,        /* out of core when the memory budget is below B (+ Bw) + metric */
+    DFJK_SUBALGO_AUTO = 0,        /* out of core when B (+ Bw) + metric + 4 GiB do not fit the free HBM */
     DFJK_SUBALGO_INCORE = 1,      /* B resident in HBM */
     DFJK_SUBALGO_OUT_OF_CORE = 2  /* B in pinned host memory, streamed through the device per compute */
 };

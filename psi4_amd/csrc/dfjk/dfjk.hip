@@ -271,13 +271,24 @@ void build_tiles(dfjk_handle* h) {
 }
 
 int g_kgemm_wgs = 2048;  // workgroups the Q split of the contraction aims at (tuning: PSI4AMD_KGEMM_WGS)
-size_t g_x_budget = size_t(4) << 30;  // bytes of X workspace without a memory setting (PSI4AMD_X_BUDGET_GB)
+size_t g_x_budget = size_t(16) << 30;       // FP64 X workspace bytes without a memory setting
+size_t g_x_budget_mixed = size_t(4) << 30;  // the same for FP32 / BF16x3 (both: PSI4AMD_X_BUDGET_GB)
+
+// Workspace bytes under the memory setting: what it leaves beyond the resident bytes, capped
+// by want.  A setting below the resident bytes sizes host memory (psi4 passes its own memory
+// option here), not this device, and leaves want unchanged.
+size_t memory_limited(const dfjk_handle* h, size_t want, size_t resident) {
+    if (!h->memory_doubles) return want;
+    const size_t mem = h->memory_doubles * sizeof(double);
+    return mem > resident ? std::min(want, mem - resident) : want;
+}
 
 void upload_tiles(dfjk_handle* h) {
     hcheck(hipMemset(h->dZero.get<double>(TILE2), 0, TILE2 * sizeof(double)), "zero tile");
     if (const char* e = std::getenv("PSI4AMD_HT_ROWWAVES")) set_ht_rowwaves(std::atoi(e));
     if (const char* e = std::getenv("PSI4AMD_KGEMM_WGS")) g_kgemm_wgs = std::max(256, std::atoi(e));
-    if (const char* e = std::getenv("PSI4AMD_X_BUDGET_GB")) g_x_budget = static_cast<size_t>(std::max(1, std::atoi(e))) << 30;
+    if (const char* e = std::getenv("PSI4AMD_X_BUDGET_GB"))
+        g_x_budget = g_x_budget_mixed = static_cast<size_t>(std::max(1, std::atoi(e))) << 30;
     const auto& sym = h->otiles_sym;
     const auto& all = h->otiles_all;
     const auto& of_sym = h->otile_of_sym;
@@ -387,7 +398,7 @@ void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vect
 std::vector<int64_t> tile_dst(const dfjk_handle* h);
 
 bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
-    if (std::getenv("PSI4AMD_HOST_INTS") || !std::getenv("PSI4AMD_DEVICE_INTS")) return false;
+    if (std::getenv("PSI4AMD_HOST_INTS")) return false;
     if (h->ooc) return false;  // a device-resident (P|mn) would not respect the memory budget
     const Basis& pb = h->prim;
     const Basis& ab = h->aux;
@@ -547,19 +558,24 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
     h->ws_G.release();
 }
 
-// Resident bytes of the in-core algorithm (B, Bw when wK is on, the metric), against which
-// AUTO decides the subalgorithm the way DFHelper::AO_core does with required_core_size_.
+// Resident doubles of the in-core algorithm (B, Bw when wK is on, the metric).  AUTO decides
+// like DFHelper::AO_core with required_core_size_ (dfhelper.cc:216-260), but against the
+// device: B goes out of core only when it and 4 GiB of workspace do not fit the free HBM.
+// (psi4's memory setting sizes host buffers; here it bounds the X workspace, not residency.)
 size_t resident_doubles(const dfjk_handle* h) {
     return b_doubles(h) * (h->do_wK ? 2 : 1) + static_cast<size_t>(h->naux) * h->naux;
 }
 
 void choose_subalgorithm(dfjk_handle* h) {
-    if (h->subalgo == DFJK_SUBALGO_OUT_OF_CORE)
+    if (h->subalgo == DFJK_SUBALGO_OUT_OF_CORE) {
         h->ooc = true;
-    else if (h->subalgo == DFJK_SUBALGO_INCORE)
+    } else if (h->subalgo == DFJK_SUBALGO_INCORE) {
         h->ooc = false;
-    else
-        h->ooc = h->memory_doubles && h->memory_doubles < resident_doubles(h);
+    } else {
+        size_t free_b = 0, total_b = 0;
+        hcheck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        h->ooc = resident_doubles(h) * sizeof(double) + (size_t(4) << 30) > free_b;
+    }
     if (h->ooc && h->do_wK)  // dfhelper.cc:201-205
         fail(DFJK_ERR_STATE, "DFJK: not equipped to do wK out of core; supply more memory or disable wK");
     if (h->ooc && h->tensor_mode) fail(DFJK_ERR_STATE, "DFJK: tensor-level handles are in core only");
@@ -790,13 +806,12 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     const size_t qs = static_cast<size_t>(h->ntp) * TILE2;  // elements per Q
     const size_t xq = static_cast<size_t>(xrows) * ldc;      // X elements per Q
     const size_t esz = prec == DFJK_PREC_FP64 ? 8 : 4;       // FP64, FP32, or BF16 hi+lo
-    size_t budget = g_x_budget;                              // bytes of X workspace
-    if (h->memory_doubles) {
-        const size_t resident =
-            (h->ooc ? 0 : b_doubles(h)) * sizeof(double) + static_cast<size_t>(h->naux) * h->naux * sizeof(double);
-        const size_t mem = h->memory_doubles * sizeof(double);
-        budget = mem > resident ? std::min(budget, mem - resident) : 0;
-    }
+    // bytes of X workspace (16 GiB FP64, 4 GiB in the mixed modes, whose contraction runs
+    // best on smaller Q-blocks), or what the memory setting leaves beyond the resident tensor
+    size_t budget = prec == DFJK_PREC_FP64 ? g_x_budget : g_x_budget_mixed;
+    budget = memory_limited(h, budget,
+                            (h->ooc ? 0 : b_doubles(h)) * sizeof(double) +
+                                static_cast<size_t>(h->naux) * h->naux * sizeof(double));
     int qblk = nq_total;
     if (doK) {
         const size_t per_q = xq * esz * (general ? 2 : 1);
@@ -1097,12 +1112,8 @@ void compute_wk_pair(dfjk_handle* h, const double* dCl, const double* dCr, int n
         launch_pack_cmap_f64(dCr, ncol, nbf, dmap, cr, nbfp, ldc, s);
     }
     const size_t xq = static_cast<size_t>(xrows) * ldc;
-    size_t budget = g_x_budget;
-    if (h->memory_doubles) {
-        const size_t resident = (2 * b_doubles(h) + static_cast<size_t>(h->naux) * h->naux) * sizeof(double);
-        const size_t mem = h->memory_doubles * sizeof(double);
-        budget = mem > resident ? std::min(budget, mem - resident) : 0;
-    }
+    const size_t budget = memory_limited(
+        h, g_x_budget, (2 * b_doubles(h) + static_cast<size_t>(h->naux) * h->naux) * sizeof(double));
     const size_t per_q = 2 * xq * sizeof(double);
     int qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
     qblk = std::min(qblk, 65535);
@@ -1337,12 +1348,7 @@ void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double*
     const size_t xq = static_cast<size_t>(xrows) * ldc;
     const size_t yq = static_cast<size_t>(nocc) * nocc, pq = static_cast<size_t>(nbf) * nocc;
     const size_t per_q = (xq + yq + pq + (gam_host ? nn : 0)) * sizeof(double);
-    size_t budget = size_t(2) << 30;  // bytes of per-Q workspace
-    if (h->memory_doubles) {
-        const size_t resident = (b_doubles(h) + 4 * nn) * sizeof(double);
-        const size_t mem = h->memory_doubles * sizeof(double);
-        budget = mem > resident ? std::min(budget, mem - resident) : 0;
-    }
+    const size_t budget = memory_limited(h, size_t(2) << 30, (b_doubles(h) + 4 * nn) * sizeof(double));
     const int nq_total = uq1 - uq0;
     int qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
     qblk = std::min(qblk, 65535);

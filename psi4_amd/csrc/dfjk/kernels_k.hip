@@ -169,7 +169,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     T ra[NA], rc[NC];
     // Column tile of the next step to load, read from the LDS step list one step
     // ahead, so a step's loads wait on one LDS lookup (the block offsets) instead of two.
-    // The FP64 5-column-block shape has no register left for it and looks the tile up in place.
+    // The FP64 5-column-block shape looks the tile up in place (measured faster there).
     constexpr bool PF = !(F64 && NBW == 5);
     int nJ = 0;
     auto fetch = [&](int st) {
@@ -594,7 +594,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
 
-    if (nsteps > 0) {
+    if constexpr (F64) {
+        // FP64: a step's operands are loaded two steps before their LDS store (two register
+        // sets named statically in a loop over step pairs; loads past the last step re-read it)
+        T sa0[8], sb0[8], sa1[8], sb1[8];
+        auto gload_at = [&](int st, T* xa, T* xb) {
+            st = min(st, nsteps - 1);
+            const size_t oa = static_cast<size_t>(b0 + st / nks) * strideA + (st % nks) * KS;
+            const size_t ob = static_cast<size_t>(b0 + st / nks) * strideB + (st % nks) * KS;
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                xa[it] = Ab[oa + static_cast<size_t>(16 * it) * ld];
+                xb[it] = Bb[ob + static_cast<size_t>(16 * it) * ld];
+            }
+        };
+        auto lstore_from = [&](int buf, const T* xa, const T* xb) {
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                As[buf][(r0 + 16 * it) * AP + kk] = xa[it];
+                Bs[buf][(r0 + 16 * it) * AP + kk] = xb[it];
+            }
+        };
+        if (nsteps > 0) {
+            gload_at(0, sa0, sb0);
+            gload_at(1, sa1, sb1);
+            lstore_from(0, sa0, sb0);
+            __syncthreads();
+            int st = 0;
+            for (; st + 1 < nsteps; st += 2) {
+                gload_at(st + 2, sa0, sb0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (!idle) compute(0);
+                lstore_from(1, sa1, sb1);
+                __syncthreads();
+                gload_at(st + 3, sa1, sb1);
+                __builtin_amdgcn_sched_barrier(0);
+                if (!idle) compute(1);
+                lstore_from(0, sa0, sb0);
+                __syncthreads();
+            }
+            if (st < nsteps && !idle) compute(0);  // odd step count: the last step is in buffer 0
+        }
+    } else if (nsteps > 0) {
         gload();
         lstore(0);
         __syncthreads();
@@ -604,11 +645,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             if (more) gload();
             if (!idle) {
                 compute(buf);
-                if constexpr (!F64) {
-                    if (++nf == FOLD) {
-                        fold();
-                        nf = 0;
-                    }
+                if (++nf == FOLD) {
+                    fold();
+                    nf = 0;
                 }
             }
             if (more) lstore(buf ^ 1);

@@ -1,4 +1,4 @@
-"""Out-of-core B (SCF_SUBTYPE OUT_OF_CORE / AUTO below the resident size): the fitted
+"""Out-of-core B (SCF_SUBTYPE OUT_OF_CORE; AUTO only when B does not fit the device): the fitted
 tensor stays in pinned host memory and each compute streams it through the device in
 Q-blocks.  On (H2O)_8 / cc-pVDZ with a memory budget that forces ~20 blocks, J and K equal
 the in-core handle's (same kernels, blocked sums: 1e-13 of the largest element) for a
@@ -14,6 +14,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _jk(mol, subalgo=None, memory=None):
+    """Both handles take (P|mn) from the host engine (out of core always does), so their
+    tensors are bit-identical and J/K differ only by the blocking."""
+    import os
     from psi4_amd.basisset import BasisSet
     from psi4_amd.core import JK
     prim = BasisSet.build(mol, "cc-pvdz")
@@ -22,7 +25,15 @@ def _jk(mol, subalgo=None, memory=None):
     jk.set_print(0)
     if subalgo:
         jk.set_subalgo(subalgo)
-    jk.initialize()
+    old = os.environ.get("PSI4AMD_HOST_INTS")
+    os.environ["PSI4AMD_HOST_INTS"] = "1"
+    try:
+        jk.initialize()
+    finally:
+        if old is None:
+            os.environ.pop("PSI4AMD_HOST_INTS")
+        else:
+            os.environ["PSI4AMD_HOST_INTS"] = old
     return jk
 
 
@@ -46,10 +57,12 @@ def test_out_of_core_matches_in_core(gpu_available):
     mol = water_cluster((2, 2, 2), seed=7)
     announce("(H2O)_8 in core")
     core = _jk(mol, "INCORE")
-    announce("(H2O)_8 out of core, ~20 Q-blocks, AUTO from the memory budget")
+    announce("(H2O)_8 out of core, ~20 Q-blocks from a small memory budget")
     naux = core.info()["naux"]
-    ooc = _jk(mol, "AUTO", memory=5_000_000)
+    ooc = _jk(mol, "OUT_OF_CORE", memory=5_000_000)
     assert ooc.info()["out_of_core"] and not core.info()["out_of_core"]
+    # AUTO keeps B in HBM whenever it fits, whatever the (host-sized) memory setting
+    assert not _jk(mol, "AUTO", memory=5_000_000).info()["out_of_core"]
     assert "Host-stream" in ooc.print_header()
     assert np.array_equal(ooc.get_B_dense(0, naux), core.get_B_dense(0, naux))
     n = core.info()["nbf"]
