@@ -82,6 +82,13 @@ int dfjk_set_omp_nthread(dfjk_handle* h, int n);           /* JK::set_omp_nthrea
 int dfjk_set_do_wK(dfjk_handle* h, int do_wK);             /* JK::set_do_wK (jk.h:459) */
 int dfjk_set_subalgorithm(dfjk_handle* h, int subalgo);    /* enum dfjk_subalgorithm (DFHelper subalgo_) */
 int dfjk_set_omega(dfjk_handle* h, double omega);          /* JK::set_omega (jk.h:473): erf(omega r)/r */
+/* JK::set_wcombine (jk.h:463-467; MemDFJK.cc:83-88, 149-154): combined exchange.  With do_wK,
+ * K is not built (its output is zeroed unless accumulating, dfhelper.cc:3022) and wK returns
+ * omega_alpha * K + omega_beta * wK(omega) (dfhelper.cc:659-667, 1411, 3378-3438).  Effective at
+ * the next wK compute; a tensor-level handle must set the attenuated tensor after it (and after B). */
+int dfjk_set_wcombine(dfjk_handle* h, int wcombine);
+int dfjk_set_omega_alpha(dfjk_handle* h, double alpha);   /* JK::set_omega_alpha (jk.h:478-481), default 1 */
+int dfjk_set_omega_beta(dfjk_handle* h, double beta);     /* JK::set_omega_beta (jk.h:485-488), default 0 */
 int dfjk_set_precision(dfjk_handle* h, int precision);     /* enum dfjk_precision (no psi4 counterpart) */
 int dfjk_set_fitting(dfjk_handle* h, int fitting);         /* enum dfjk_fitting */
 int dfjk_set_stream(dfjk_handle* h, void* hip_stream);     /* NULL: the handle's own stream */
@@ -116,7 +123,9 @@ int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const do
 /* JK::wK() of JK::compute with do_wK (jk.cc:382-423) -> DFHelper::compute_wK
  * (dfhelper.cc:3378-3438), MemDFJK's non-combined path (wcombine off, MemDFJK.cc:81-90):
  *   wK = sum_Q (B_Q Cl)(Bw_Q Cr)^T = sum_{PQ} ((P|mn) Cl) [J^-1]_PQ ((Q|erf(omega r)/r|mn) Cr)^T,
- * FP64 whatever the precision setting; Cr == NULL means Cr = Cl.  Not symmetric in general.
+ * or omega_alpha K + omega_beta of that with wcombine.  FP64 whatever the precision setting.
+ * Cr == NULL means Cr = Cl (lr_symmetric): wK is then hermitivitized, (wK + wK^T)/2, as
+ * MemDFJK::compute_JK does (MemDFJK.cc:101-107); otherwise it is not symmetric in general.
  * wK[i]: nbf x nbf, overwritten unless accumulate != 0. */
 int dfjk_compute_wK(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
                     double* const* wK, int accumulate);

@@ -209,6 +209,16 @@ def compute_wK(m1Ppq, wPpq, sp: Sparsity, Cl, Cr, wK):
     wK += T1 @ T2.T
 
 
+def hermitivitize(A):
+    """Matrix::hermitivitize (libmints/matrix.cc:2681-2700): A_rc = A_cr = (A_rc + A_cr) / 2."""
+    A = np.array(A, dtype=np.float64)
+    iu = np.triu_indices(A.shape[0], 1)
+    v = 0.5 * (A[iu] + A.T[iu])
+    A[iu] = v
+    A.T[iu] = v
+    return A
+
+
 def build_JK(Ppq, sp: Sparsity, C_left, C_right=None, do_J=True, do_K=True):
     """JK::compute (libfock/jk.cc:595-681) + MemDFJK::compute_JK (MemDFJK.cc:97-111)
     + DFHelper::compute_JK (dfhelper.cc:3044-3161) with one Q block.
@@ -248,12 +258,16 @@ class MemDFJKOracle:
     can be checked on identical B."""
 
     def __init__(self, primary, aux, cutoff=1e-12, condition=1e-10, nthreads=None, fit="eigen", B=None,
-                 omega=None, Bw=None):
+                 omega=None, Bw=None, wcombine=False, omega_alpha=1.0, omega_beta=0.0):
         from psi4_amd import ints  # the Libint2 stand-in (host integral engine)
         self.primary, self.aux = primary, aux
         self.cutoff, self.condition = cutoff, condition
         self.nbf, self.naux = primary.nbf(), aux.nbf()
         self.omega = omega
+        # combined exchange (MemDFJK.cc:83-88): only with wK; K is then not built and wK
+        # pairs J^-1 (Q|mn) with alpha (Q|mn) + beta (Q|w|mn) (dfhelper.cc:659-667, 1411)
+        self.wcombine = bool(wcombine) and (omega is not None or Bw is not None)
+        a, b = float(omega_alpha), float(omega_beta)
         fm, sm, mx = ints.schwarz(primary, nthreads)
         self.sp = prepare_sparsity(fm, sm, mx, cutoff, self.naux)
         self.m1Ppq = self.wPpq = None
@@ -265,7 +279,8 @@ class MemDFJKOracle:
                 # sides of compute_wK carry one factor M each: sum_Q (B_Q Cl)(Bw_Q Cr)^T equals
                 # psi4's (J^-1 (P|mn) Cl)((P|w|mn) Cr)^T
                 self.m1Ppq = self.Ppq
-                self.wPpq = ppq_from_dense(np.asarray(Bw), self.sp)
+                Bw = np.asarray(Bw)
+                self.wPpq = ppq_from_dense(a * np.asarray(B) + b * Bw if self.wcombine else Bw, self.sp)
             return
         self.metric = ints.metric(aux, nthreads)
         if fit == "eigen":
@@ -291,6 +306,8 @@ class MemDFJKOracle:
             self.m1Ppq = build_ppq(self.Pmn, met1, self.sp)
             W = ints.three_index(primary, aux, nthreads, omega=omega)
             W[:, ~self.sp.fun_mask] = 0.0
+            if self.wcombine:
+                W = a * self.Pmn + b * W
             self.wPpq = build_ppq(W, np.eye(self.naux), self.sp)
 
     def memory_estimate(self, nthreads=1):
@@ -300,17 +317,21 @@ class MemDFJKOracle:
         return ppq_to_dense(self.Ppq, self.sp)
 
     def compute(self, C_left, C_right=None, do_J=True, do_K=True):
-        return build_JK(self.Ppq, self.sp, C_left, C_right, do_J, do_K)
+        # dfhelper.cc:3022: with the exchange combined into wK, K is not built (zeros)
+        return build_JK(self.Ppq, self.sp, C_left, C_right, do_J, do_K and not self.wcombine)
 
     def compute_wK(self, C_left, C_right=None):
         """wK of every pair (DFHelper::build_JK -> compute_wK, dfhelper.cc:3032-3035)."""
         if self.wPpq is None:
             raise ValueError("oracle built without omega / Bw")
-        C_right = C_left if C_right is None else C_right
+        lr_symmetric = C_right is None
+        C_right = C_left if lr_symmetric else C_right
         out = []
         for Cl, Cr in zip(C_left, C_right):
             wK = np.zeros((self.nbf, self.nbf))
             compute_wK(self.m1Ppq, self.wPpq, self.sp, Cl, Cr, wK)
+            if lr_symmetric:
+                wK = hermitivitize(wK)           # MemDFJK::compute_JK, MemDFJK.cc:101-107
             out.append(wK)
         return out
 

@@ -186,6 +186,12 @@ struct dfjk_handle {
     // range-separated exchange: Bw = M (P|erf(omega r)/r|mn), same layout and fit as B
     DevBuf dBw;
     double bw_omega = -1.0;         // omega dBw was built for (< 0: none)
+    // combined exchange (wcombine): dBw then holds alpha B + beta Bw, so that the wK build
+    // returns alpha K + beta wK; bw_* record what dBw holds
+    bool wcombine = false;
+    double omega_alpha = 1.0, omega_beta = 0.0;
+    bool bw_comb = false;
+    double bw_alpha = 1.0, bw_beta = 0.0;
     std::vector<double> fit_M;      // host metric factor, kept for rebuilding Bw after an omega change
     // workspace
     DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G, ws_sign, ws_c32, ws_cbf, ws_cmap;
@@ -608,6 +614,28 @@ std::vector<int64_t> tile_dst(const dfjk_handle* h) {
 // (P|erf(omega r)/r|mn) on the host, fitted with the Coulomb metric factor into dBw.  psi4 keeps
 // the raw attenuated tensor and contracts the full inverse metric into the other side
 // (dfhelper.cc:589-699, 3378-3438); with J^-1 = M^T M both sides carry one factor M here.
+// dBw matches the current omega and exchange-combination settings
+bool bw_current(const dfjk_handle* h) {
+    if (h->bw_omega != h->omega || h->bw_comb != h->wcombine) return false;
+    return !h->wcombine || (h->bw_alpha == h->omega_alpha && h->bw_beta == h->omega_beta);
+}
+
+// Combined exchange (dfhelper.cc:659-667, 1411): psi4 pairs J^-1 (Q|mn) with
+// alpha (Q|mn) + beta (Q|w|mn); with one factor M on each side that is alpha B + beta Bw,
+// formed in place in dBw.
+void combine_Bw(dfjk_handle* h) {
+    h->bw_comb = false;
+    if (h->wcombine) {
+        if (h->ooc) fail(DFJK_ERR_STATE, "wK is not available with B out of core");
+        launch_axpby_mat(h->dB.as<double>(), h->omega_alpha, h->dBw.as<double>(), h->omega_beta,
+                         h->dBw.as<double>(), b_doubles(h), h->stream);
+        hcheck(hipStreamSynchronize(h->stream), "combine wK tensor");
+        h->bw_comb = true;
+        h->bw_alpha = h->omega_alpha;
+        h->bw_beta = h->omega_beta;
+    }
+}
+
 void build_Bw(dfjk_handle* h) {
     if (h->tensor_mode) fail(DFJK_ERR_STATE, "wK needs the attenuated tensor: dfjk_set_Bw_dense");
     if (!(h->omega > 0.0)) fail(DFJK_ERR_ARG, "wK needs omega > 0 (dfjk_set_omega)");
@@ -624,6 +652,7 @@ void build_Bw(dfjk_handle* h) {
         fit_on_device(h, h->fit_M, A, h->dBw);
     }
     h->bw_omega = h->omega;
+    combine_Bw(h);
 }
 
 void initialize_basis(dfjk_handle* h) {
@@ -1182,6 +1211,17 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
         double* dD = D ? D[i] : nullptr;
         double* dJ = J ? J[i] : nullptr;
         double* dK = K ? K[i] : nullptr;
+        if (dK && h->do_wK && h->wcombine) {
+            // combined exchange: K is not built (dfhelper.cc:3022) and stays as MemDFJK::compute_JK's
+            // zero() left it; the exchange comes back in wK
+            if (!accumulate) {
+                if (device)
+                    hcheck(hipMemsetAsync(dK, 0, nn * sizeof(double), h->stream), "memset");
+                else
+                    std::memset(dK, 0, nn * sizeof(double));
+            }
+            dK = nullptr;
+        }
         if (ncol[i] == 0) {
             // psi4: D = 0, J += 0, K skipped for nocc == 0 (dfhelper.cc:3355-3358)
             if (device) {
@@ -1236,7 +1276,9 @@ void compute_wk_impl(dfjk_handle* h, int n, const double* const* Cl, const doubl
     if (n < 0 || (n > 0 && (!Cl || !ncol || !wK))) fail(DFJK_ERR_ARG, "JK: bad wK arguments");
     if (h->tensor_mode) {
         if (!h->dBw.p) fail(DFJK_ERR_STATE, "wK needs the attenuated tensor: dfjk_set_Bw_dense");
-    } else if (h->bw_omega != h->omega) {
+        if (h->bw_comb != h->wcombine || (h->wcombine && (h->bw_alpha != h->omega_alpha || h->bw_beta != h->omega_beta)))
+            fail(DFJK_ERR_STATE, "wcombine / omega_alpha / omega_beta changed after dfjk_set_Bw_dense: set it again");
+    } else if (!bw_current(h)) {
         build_Bw(h);
     }
     reset_timers(h);
@@ -1273,6 +1315,8 @@ void compute_wk_impl(dfjk_handle* h, int n, const double* const* Cl, const doubl
             if (accumulate) hcheck(hipMemcpyAsync(gW, dW, nn * sizeof(double), hipMemcpyHostToDevice, h->stream), "up");
         }
         compute_wk_pair(h, cl, cr, ncol[i], gW, accumulate);
+        // lr_symmetric: MemDFJK::compute_JK hermitivitizes wK (MemDFJK.cc:101-107)
+        if (!cr) launch_hermitivitize(gW, h->nbf, h->nbf, h->stream);
         if (!device) {
             hcheck(hipMemcpyAsync(dW, gW, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download wK");
             hcheck(hipStreamSynchronize(h->stream), "compute wK");
@@ -1488,6 +1532,26 @@ int dfjk_set_subalgorithm(dfjk_handle* h, int v) {
         h->subalgo = v;
     });
 }
+int dfjk_set_wcombine(dfjk_handle* h, int wcombine) {
+    return guarded([&] {
+        require_state(h, false);
+        h->wcombine = wcombine != 0;  // takes effect at the next wK compute
+    });
+}
+int dfjk_set_omega_alpha(dfjk_handle* h, double alpha) {
+    return guarded([&] {
+        require_state(h, false);
+        if (!std::isfinite(alpha)) fail(DFJK_ERR_ARG, "omega_alpha must be finite");
+        h->omega_alpha = alpha;
+    });
+}
+int dfjk_set_omega_beta(dfjk_handle* h, double beta) {
+    return guarded([&] {
+        require_state(h, false);
+        if (!std::isfinite(beta)) fail(DFJK_ERR_ARG, "omega_beta must be finite");
+        h->omega_beta = beta;
+    });
+}
 int dfjk_set_omega(dfjk_handle* h, double omega) {
     return guarded([&] {
         require_state(h, false);
@@ -1660,8 +1724,10 @@ int dfjk_set_B_dense(dfjk_handle* h, const double* B, int on_device) {
 
 int dfjk_set_Bw_dense(dfjk_handle* h, const double* Bw, int on_device) {
     return guarded([&] {
+        if (h && h->wcombine) require_state(h, true);  // combined with B on arrival
         set_dense(h, Bw, on_device, h->dBw);
         h->bw_omega = h->omega;
+        combine_Bw(h);
     });
 }
 
@@ -1689,7 +1755,7 @@ int dfjk_get_B_dense(dfjk_handle* h, int q0, int q1, double* out) {
 int dfjk_get_Bw_dense(dfjk_handle* h, int q0, int q1, double* out) {
     return guarded([&] {
         require_state(h, true);
-        if (!h->tensor_mode && h->bw_omega != h->omega) build_Bw(h);
+        if (!h->tensor_mode && !bw_current(h)) build_Bw(h);
         get_dense(h, h->dBw, q0, q1, out);
     });
 }

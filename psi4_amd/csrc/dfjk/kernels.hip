@@ -415,10 +415,21 @@ void launch_gemm_nn_f64(const double* A, int lda, const double* B, int ldb, doub
 }
 
 // ---- adjoint helpers ----
-__global__ void k_axpby_mat(const double* __restrict__ X, double a, const double* __restrict__ Y, double b,
-                            double* __restrict__ Z, size_t n) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) Z[i] = a * X[i] + b * Y[i];
+__global__ void k_axpby_mat(const double* __restrict__ X, double a, const double* Y, double b, double* Z, size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        Z[i] = a * X[i] + b * Y[i];
+}
+// A = (A + A^T) / 2 in place (Matrix::hermitivitize, matrix.cc:2681-2700): the thread of
+// (r, c), c > r, owns the pair
+__global__ void k_hermitivitize(double* __restrict__ A, int n, int lda) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (c > r && c < n) {
+        const size_t rc = static_cast<size_t>(r) * lda + c, cr = static_cast<size_t>(c) * lda + r;
+        const double v = 0.5 * (A[rc] + A[cr]);
+        A[rc] = v;
+        A[cr] = v;
+    }
 }
 // partial[b] = sum over the block's slice of X.*Y (fixed tree, deterministic)
 __global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ X, const double* __restrict__ Y,
@@ -452,7 +463,13 @@ __global__ void k_fill_scaled(const double* __restrict__ d, int q0, double a, co
 }
 
 void launch_axpby_mat(const double* X, double a, const double* Y, double b, double* Z, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_axpby_mat, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, X, a, Y, b, Z, n);
+    const size_t nb = std::min<size_t>((n + 255) / 256, size_t(1) << 20);
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_axpby_mat, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, X, a, Y, b, Z, n);
+}
+void launch_hermitivitize(double* A, int n, int lda, hipStream_t s) {
+    require(n >= 1 && n <= 65535 && lda >= n, "hermitivitize");
+    hipLaunchKernelGGL(k_hermitivitize, dim3((n + 255) / 256, n), dim3(256), 0, s, A, n, lda);
 }
 void launch_dot_partial(const double* X, const double* Y, size_t n, double* partial, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_dot_partial, dim3(nblocks), dim3(256), 0, s, X, Y, n, partial);

@@ -66,6 +66,8 @@ void launch_gemm_nn_f64(const double* A, int lda, const double* B, int ldb, doub
 // ---- adjoint helpers ----
 // Z = a X + b Y (n elements)
 void launch_axpby_mat(const double* X, double a, const double* Y, double b, double* Z, size_t n, hipStream_t s);
+// A = (A + A^T) / 2 in place (n x n, leading dimension lda)
+void launch_hermitivitize(double* A, int n, int lda, hipStream_t s);
 // partial[0..nblocks) = block sums of X.*Y (deterministic; sum the partials on the host)
 void launch_dot_partial(const double* X, const double* Y, size_t n, double* partial, int nblocks, hipStream_t s);
 // AT[c][r] = A[r][c]

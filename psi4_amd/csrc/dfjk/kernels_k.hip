@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "kernels_k.hpp"
@@ -222,12 +223,16 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     // fragment of the next column block are read while the current block's MFMAs run, so
     // the LDS reads overlap the MFMAs (consecutive updates of one accumulator are 4*NBW
     // MFMAs apart).
+    // The last row block can reach past the last row tile: a wave's 16-row fragments there
+    // keep zero accumulators, and their A reads and MFMAs are skipped (NI = fragments kept).
     constexpr int NSUB = KS / 4;
-    auto compute = [&](int buf) {
+    const int nfr = __builtin_amdgcn_readfirstlane(min(4, max(0, 2 * (nT - I0 - 2 * wm))));
+    auto compute = [&](int buf, auto ni) {
+        constexpr int NI = decltype(ni)::value;
         T a[2][4], b[2];
         auto aread = [&](int u, int slot) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < NI; ++i)
                 a[slot][i] = As[buf][(wm * 64 + i * 16 + (lane & 15)) * AP + 4 * u + (lane >> 4)];
         };
         auto bread = [&](int u, int j, int slot) {
@@ -247,7 +252,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
                 if (j == 0 && u + 1 < NSUB) aread(u + 1, (u + 1) & 1);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < NI; ++i) {
                     if constexpr (F64)
                         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u & 1][i], b[t & 1], acc[i][j], 0, 0, 0);
                     else
@@ -258,7 +263,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         }
     };
 
-    if (nsteps > 0) {
+    auto mainloop = [&](auto ni) {
         fetch(0);
         gload(0);
         lstore(0, 0);
@@ -268,11 +273,19 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             const bool more = st + 1 < nsteps;
             if (more) gload(st + 1);
             __builtin_amdgcn_sched_barrier(0);
-            compute(buf);
+            if constexpr (decltype(ni)::value > 0) compute(buf, ni);
             if (more) lstore(buf ^ 1, st + 1);
             __syncthreads();
             buf ^= 1;
         }
+    };
+    if (nsteps > 0) {
+        if (nfr == 4)
+            mainloop(std::integral_constant<int, 4>{});
+        else if (nfr == 2)
+            mainloop(std::integral_constant<int, 2>{});
+        else
+            mainloop(std::integral_constant<int, 0>{});
     }
     // every row of the workgroup's block is written (absent row tiles give zeros)
     T* Xq = X + static_cast<size_t>(q) * xrows * ldc;
@@ -420,7 +433,11 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     // One step's products: all fragments first (one LDS wait), then the MFMAs product by
     // product (consecutive updates of one accumulator are 4*NBW MFMAs apart), then the next
     // step's staging into the other buffer.
-    auto compute = [&](int buf, auto&& store_next) {
+    // A wave whose 16-row fragments lie past the last row tile (last row block only) skips
+    // their fragment reads and MFMAs; their accumulators stay zero.
+    const int nfr = __builtin_amdgcn_readfirstlane(min(4, max(0, 2 * (nT - I0 - 2 * wm))));
+    auto compute = [&](int buf, auto ni, auto&& store_next) {
+        constexpr int NI = decltype(ni)::value;
         const uint16_t* Ah = Ls[buf];
         const uint16_t* Al = Ah + A_SZ;
         const uint16_t* Ch = Al + A_SZ;
@@ -428,7 +445,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const int fc = lane >> 4;
         bf8_t ah[4], al[4], bh[NBW], bl[NBW];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < NI; ++i) {
             const int o = kg_swz(wm * 64 + i * 16 + (lane & 15), fc);
             ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ah + o));
             al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + o));
@@ -442,22 +459,22 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < NI; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < NI; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < NI; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
         __builtin_amdgcn_sched_barrier(0);
         store_next();
     };
     // Steps run in pairs so the two A sets are named statically; the loop body has no
     // branches around its loads and stores (loads past the last step re-read it, and their
     // stores land in a buffer no later compute reads).
-    if (nsteps > 0) {
+    auto mainloop = [&](auto ni) {
         gloadC(0);
         gloadA(0, a0h, a0l);
         lstore(0, a0h, a0l);
@@ -469,15 +486,23 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             gloadC(st + 1);
             gloadA(st + 2, a0h, a0l);
             __builtin_amdgcn_sched_barrier(0);
-            compute(0, [&] { lstore(1, a1h, a1l); });
+            compute(0, ni, [&] { lstore(1, a1h, a1l); });
             __syncthreads();
             gloadC(st + 2);
             gloadA(st + 3, a1h, a1l);
             __builtin_amdgcn_sched_barrier(0);
-            compute(1, [&] { lstore(0, a0h, a0l); });
+            compute(1, ni, [&] { lstore(0, a0h, a0l); });
             __syncthreads();
         }
-        if (st < nsteps) compute(0, [] {});  // odd step count: the last step is in buffer 0
+        if (st < nsteps) compute(0, ni, [] {});  // odd step count: the last step is in buffer 0
+    };
+    if (nsteps > 0) {
+        if (nfr == 4)
+            mainloop(std::integral_constant<int, 4>{});
+        else if (nfr == 2)
+            mainloop(std::integral_constant<int, 2>{});
+        else
+            mainloop(std::integral_constant<int, 0>{});
     }
     const size_t xq = static_cast<size_t>(q) * xrows * ldc;
 #pragma unroll

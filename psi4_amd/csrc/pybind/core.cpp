@@ -83,7 +83,6 @@ class JK {
         return d;
     }
     void initialize() {
-        if (wcombine_) throw std::runtime_error("MemDFJK on MI355X: wcombine is not supported");
         check(dfjk_initialize(h_));
         initialized_ = true;
     }
@@ -109,11 +108,12 @@ class JK {
     }
     void set_omega(double w) { check(dfjk_set_omega(h_, w)); omega_ = w; }
     double get_omega() const { return omega_; }
-    void set_wcombine(bool w) { wcombine_ = w; }
+    // combined exchange (MemDFJK.cc:83-90, 140-154): with do_wK, wK = alpha K + beta wK(omega), K not built
+    void set_wcombine(bool w) { check(dfjk_set_wcombine(h_, w)); wcombine_ = w; }
     bool get_wcombine() const { return wcombine_; }
-    void set_omega_alpha(double a) { omega_alpha_ = a; }
+    void set_omega_alpha(double a) { check(dfjk_set_omega_alpha(h_, a)); omega_alpha_ = a; }
     double get_omega_alpha() const { return omega_alpha_; }
-    void set_omega_beta(double b) { omega_beta_ = b; }
+    void set_omega_beta(double b) { check(dfjk_set_omega_beta(h_, b)); omega_beta_ = b; }
     double get_omega_beta() const { return omega_beta_; }
     void set_condition(double c) { check(dfjk_set_condition(h_, c)); condition_ = c; }
     void set_precision(int p) { check(dfjk_set_precision(h_, p)); precision_ = p; }

@@ -24,7 +24,7 @@ import time
 from dataclasses import dataclass, field
 
 import numpy as np
-from scipy.linalg import eigh
+import torch
 
 from . import ints
 from .core import JK, PREC_BF16X3, PREC_FP32, PREC_FP64
@@ -50,16 +50,26 @@ class SCFResult:
     timings: dict = field(default_factory=dict)
 
 
+def _vdot(a, b):
+    return float(torch.sum(a * b)) if torch.is_tensor(a) else float(np.vdot(a, b))
+
+
+def _copy(a):
+    return a.clone() if torch.is_tensor(a) else a.copy()
+
+
 class _DIIS:
+    """Pulay DIIS over numpy arrays or torch tensors (host or device)."""
+
     def __init__(self, max_vecs):
         self.max_vecs = max_vecs
         self.F, self.err = [], []
 
     def add(self, F, err):
-        self.F.append(F.copy())
-        self.err.append(err.copy())
+        self.F.append(_copy(F))
+        self.err.append(_copy(err))
         if len(self.F) > self.max_vecs:
-            worst = int(np.argmax([np.linalg.norm(e) for e in self.err]))
+            worst = int(np.argmax([_vdot(e, e) for e in self.err]))
             del self.F[worst]
             del self.err[worst]
 
@@ -69,7 +79,7 @@ class _DIIS:
         B[n, n] = 0.0
         for i in range(n):
             for j in range(i, n):
-                B[i, j] = B[j, i] = np.vdot(self.err[i], self.err[j])
+                B[i, j] = B[j, i] = _vdot(self.err[i], self.err[j])
         rhs = np.zeros(n + 1)
         rhs[n] = -1.0
         c = np.linalg.lstsq(B, rhs, rcond=None)[0]
@@ -132,45 +142,76 @@ class RHF:
         from . import response
         return response.cphf_solve(self, x_vec, conv_tol, max_iter, print_lvl)
 
-    def _full_build(self, Cocc):
+    def _device(self):
+        """Where the SCF algebra runs: on the GPU next to the engine's J/K when the JK is the
+        device engine, else on the host (a JK object that takes and returns numpy arrays)."""
         jk = self.jk()
-        jk.set_precision(PREC_FP64)
-        jk.C_clear()
-        jk.C_left_add(Cocc)
-        jk.compute()
-        return jk.J()[0].copy(), jk.K()[0].copy()
+        if hasattr(jk, "compute_device") and torch.cuda.is_available():
+            return torch.device("cuda", torch.cuda.current_device())
+        return torch.device("cpu")
 
-    def _delta_build(self, Cnew, Cold, S):
+    def _jk_call(self, prec, Cl, Cr=None):
+        """J, K of one density pair (C_left, C_right) as tensors on the SCF device."""
+        jk = self.jk()
+        jk.set_precision(prec)
+        try:
+            if Cl.is_cuda:
+                nbf, ncol = Cl.shape
+                J = torch.empty((nbf, nbf), dtype=torch.float64, device=Cl.device)
+                K = torch.empty_like(J)
+                torch.cuda.synchronize()       # C ready before the engine's stream reads it
+                jk.compute_device([Cl.data_ptr()], [] if Cr is None else [Cr.data_ptr()], [ncol], [],
+                                  [J.data_ptr()], [K.data_ptr()], False)
+                torch.cuda.synchronize()       # the engine's stream is done with J, K
+                return J, K
+            jk.C_clear()
+            jk.C_left_add(Cl.numpy())
+            if Cr is not None:
+                jk.C_right_add(Cr.numpy())
+            jk.compute()
+            return torch.from_numpy(jk.J()[0].copy()), torch.from_numpy(jk.K()[0].copy())
+        finally:
+            jk.set_precision(PREC_FP64)
+
+    def _full_build(self, Cocc):
+        return self._jk_call(PREC_FP64, Cocc)
+
+    def _delta_build(self, Cnew, Cold, SCnew):
         """J(dD), K(dD) for dD = Cnew Cnew^T - Cold Cold^T in the cross form
         dD = A V^T + V A^T, A = (Cnew + Cold)/2, V = Cnew - Cold (C_left = [A, V],
         C_right = [V, A]), after aligning Cold to Cnew: every low-precision product
-        carries the small factor V, and the engine keeps K(dD) symmetric."""
-        u, _, vt = np.linalg.svd(Cold.T @ S @ Cnew)
-        Col = Cold @ (u @ vt)            # same density, aligned with Cnew
+        carries the small factor V, and the engine keeps K(dD) symmetric.  SCnew = S Cnew."""
+        M = (Cold.T @ SCnew).cpu().numpy()
+        u, _, vt = np.linalg.svd(M)      # orthogonal Procrustes rotation (nocc x nocc, on the host)
+        Col = Cold @ torch.from_numpy(u @ vt).to(Cold.device)   # same density, aligned with Cnew
         V = Cnew - Col
         A = 0.5 * (Cnew + Col)
-        jk = self.jk()
-        jk.set_precision(_PREC[self.mixed])
-        jk.C_clear()
-        jk.C_left_add(np.ascontiguousarray(np.hstack([A, V])))
-        jk.C_right_add(np.ascontiguousarray(np.hstack([V, A])))
-        jk.compute()
-        jk.set_precision(PREC_FP64)
-        return jk.J()[0].copy(), jk.K()[0].copy()
+        return self._jk_call(_PREC[self.mixed], torch.cat([A, V], 1).contiguous(),
+                             torch.cat([V, A], 1).contiguous())
 
     def compute_energy(self):
         t0 = time.perf_counter()
         mol, bs = self.mol, self.basis
-        S = ints.overlap(bs)
-        H = ints.kinetic(bs) + ints.potential(bs, mol)
+        S_h = ints.overlap(bs)
+        H_h = ints.kinetic(bs) + ints.potential(bs, mol)
         if self.perturb_dipole is not None:
-            H = H - np.einsum("d,dmn->mn", self.perturb_dipole, ints.dipole(bs))
+            H_h = H_h - np.einsum("d,dmn->mn", self.perturb_dipole, ints.dipole(bs))
         enuc = mol.nuclear_repulsion_energy()
-        w, U = np.linalg.eigh(S)
-        X = (U * w ** -0.5) @ U.T
+        dev = self._device()
+        S = torch.from_numpy(S_h).to(dev)
+        H = torch.from_numpy(H_h).to(dev)
+        # symmetric orthogonalizer X = S^-1/2; F C = S C eps is solved as (X^T F X) C' = C' eps,
+        # C = X C' (hf.cc form_C with the symmetric orthogonalization)
+        w, U = torch.linalg.eigh(S)
+        X = (U * w.rsqrt()) @ U.T
+
+        def diag(F):
+            e, Cp = torch.linalg.eigh(X.T @ F @ X)
+            return e, X @ Cp
+
         self.jk()
         t_setup = time.perf_counter() - t0
-        eps, C = eigh(H, S)
+        eps, C = diag(H)
         diis = _DIIS(self.diis_max_vecs)
         E_old = 0.0
         history = []
@@ -180,7 +221,7 @@ class RHF:
         mixed_on = self.mixed is not None
         t_jk = 0.0
         for it in range(1, self.maxiter + 1):
-            Cocc = np.ascontiguousarray(C[:, :self.nocc])
+            Cocc = C[:, :self.nocc].contiguous()
             D = Cocc @ Cocc.T
             tj = time.perf_counter()
             if J is None or not mixed_on or since_full + 1 >= self.full_every:
@@ -188,7 +229,7 @@ class RHF:
                 kind = "fp64"
                 since_full = 0
             else:
-                dJ, dK = self._delta_build(Cocc, C_prev, S)
+                dJ, dK = self._delta_build(Cocc, C_prev, S @ Cocc)
                 J = J + dJ
                 K = K + dK
                 kind = self.mixed
@@ -196,27 +237,30 @@ class RHF:
             dt_jk = time.perf_counter() - tj
             t_jk += dt_jk
             if self.jk_callback is not None:
-                self.jk_callback(it, Cocc, J, K, kind)
+                self.jk_callback(it, Cocc.cpu().numpy(), J.cpu().numpy(), K.cpu().numpy(), kind)
             F = H + 2.0 * J - K
-            E = enuc + 2.0 * np.vdot(D, H) + np.vdot(D, 2.0 * J - K)
-            grad = X.T @ (F @ D @ S - S @ D @ F) @ X
-            d_rms = float(np.sqrt(np.mean(grad ** 2)))
+            E = enuc + float(torch.sum(D * (2.0 * H + 2.0 * J - K)))
+            FDS = F @ D @ S
+            grad = X.T @ (FDS - FDS.T) @ X
+            d_rms = float(torch.sqrt(torch.mean(grad * grad)))
             history.append(dict(iteration=it, energy=E, d_rms=d_rms, build=kind, jk_s=dt_jk))
             converged = abs(E - E_old) < self.e_conv and d_rms < self.d_conv
             if converged and kind == "fp64":
-                self.Ca, self.Fa, self.eps = C, F, eps
-                return SCFResult(E, C, eps, D, J, K, F, it, history,
-                                 dict(setup_s=t_setup, jk_s=t_jk, total_s=time.perf_counter() - t0))
+                return self._result(E, C, eps, D, J, K, F, it, history, t_setup, t_jk, t0)
             if mixed_on and d_rms < self.mixed_until:
                 mixed_on = False         # FP64 finishing iterations from here on
             diis.add(F, grad)
-            eps, C = eigh(diis.extrapolate(), S)
+            eps, C = diag(diis.extrapolate())
             C_prev = Cocc
             E_old = E
         if self.fail_on_maxiter:
             raise SCFConvergenceError(f"SCF did not converge in {self.maxiter} iterations")
-        return SCFResult(E, C, eps, D, J, K, F, self.maxiter, history,
-                         dict(setup_s=t_setup, jk_s=t_jk, total_s=time.perf_counter() - t0))
+        return self._result(E, C, eps, D, J, K, F, self.maxiter, history, t_setup, t_jk, t0)
+
+    def _result(self, E, C, eps, D, J, K, F, it, history, t_setup, t_jk, t0):
+        h = [x.cpu().numpy() for x in (C, eps, D, J, K, F)]
+        self.Ca, self.Fa, self.eps = h[0], h[5], h[1]
+        return SCFResult(E, *h, it, history, dict(setup_s=t_setup, jk_s=t_jk, total_s=time.perf_counter() - t0))
 
 
 def energy(mol, basis="cc-pvdz", jk=None, **kw):

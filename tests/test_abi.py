@@ -72,9 +72,15 @@ def test_jk_build_host_logic():
     assert jk.memory_estimate() - est == 464 * jk.info()["ntile_pairs"] * 1024
     with pytest.raises(RuntimeError, match="omega"):
         jk.set_omega(-1.0)
-    jk.set_wcombine(True)
-    with pytest.raises(RuntimeError, match="wcombine"):
-        jk.initialize()
+    jk.set_wcombine(True)               # combined exchange: argument checks of the weights
+    assert jk.get_wcombine()
+    with pytest.raises(RuntimeError, match="omega_alpha"):
+        jk.set_omega_alpha(float("nan"))
+    with pytest.raises(RuntimeError, match="omega_beta"):
+        jk.set_omega_beta(float("inf"))
+    jk.set_omega_alpha(0.25)
+    jk.set_omega_beta(0.75)
+    assert (jk.get_omega_alpha(), jk.get_omega_beta()) == (0.25, 0.75)
     jk.set_wcombine(False)
     jk.set_do_wK(False)
     with pytest.raises(RuntimeError, match="initialize"):

@@ -88,7 +88,7 @@ def test_autograd_vs_central_differences_h2o16(h2o16):
     # 4(2J - K)C from the oracle's MemDFJK contractions on the same B
     _, J, K = MemDFJKOracle(solver.basis, solver.aux, B=B).compute([C])
     ref = 4.0 * (2.0 * J[0] - K[0]) @ C
-    print(f"  E {float(E):.12f}  dE/dC vs 4(2J-K)C rel {_rel(gC, ref):.2e}", flush=True)
+    print(f"  E {float(E.detach()):.12f}  dE/dC vs 4(2J-K)C rel {_rel(gC, ref):.2e}", flush=True)
     assert _rel(gC, ref) < 1e-12
 
     rng = np.random.default_rng(20260116)

@@ -6,7 +6,11 @@
   (1e-11 of the largest element), and K is unchanged by do_wK;
 * (H2O)_8 after an omega change (the tensor is rebuilt): wK against the oracle built from
   the host integrals with psi4's J^-1 route (Cholesky metric, 1e-10);
-* omega -> infinity: the device wK equals the device K (1e-8).
+* omega -> infinity: the device wK equals the device K (1e-8);
+* combined exchange (wcombine, omega_alpha, omega_beta): wK = alpha K + beta wK(omega) against
+  the oracle's restatement of psi4's combined tensors on the device's own B and Bw (1e-11),
+  K zero and J unchanged, then a change of (alpha, beta) between computes (tensor rebuilt);
+  the symmetric-pair wK is hermitivitized like MemDFJK::compute_JK (exactly symmetric).
 """
 import numpy as np
 import pytest
@@ -99,3 +103,45 @@ def test_wk_large_omega_is_K(gpu_available):
     jk.C_left_add(C)
     jk.compute()
     assert _rel(jk.wK()[0], jk.K()[0]) < 1e-8
+
+
+def test_wk_combined_exchange(gpu_available):
+    mol = M.water_zmat(0.96, 104.5)
+    announce("wK water cc-pVDZ omega 0.4: wcombine (alpha 0.75, beta -0.4), then (0.25, 1.0)")
+    prim, aux, jk = _jk(mol, OMEGA)
+    naux, n = jk.info()["naux"], prim.nbf()
+    B = jk.get_B_dense(0, naux)
+    Bw = jk.get_Bw_dense(0, naux)
+    rng = np.random.default_rng(12)
+    C = np.linalg.qr(rng.standard_normal((n, n)))[0][:, :5]
+    Cl, Cr = rng.standard_normal((n, 6)), rng.standard_normal((n, 6))
+    jk.C_clear()
+    jk.C_left_add(Cl)
+    jk.C_right_add(Cr)
+    jk.compute()
+    J0 = jk.J()[0].copy()
+    for a, b in ((0.75, -0.4), (0.25, 1.0)):
+        jk.set_wcombine(True)
+        jk.set_omega_alpha(a)
+        jk.set_omega_beta(b)
+        ora = MemDFJKOracle(prim, aux, B=B, Bw=Bw, wcombine=True, omega_alpha=a, omega_beta=b)
+        jk.C_clear()
+        jk.C_left_add(Cl)
+        jk.C_right_add(Cr)
+        jk.compute()
+        e = _rel(jk.wK()[0], ora.compute_wK([Cl], [Cr])[0])
+        print(f"  ({a}, {b}) non-symmetric pair combined wK rel err {e:.2e}", flush=True)
+        assert e < 1e-11
+        assert not jk.K()[0].any()
+        assert _rel(jk.J()[0], J0) < 1e-14
+        jk.C_clear()
+        jk.C_left_add(C)
+        jk.compute()
+        w = jk.wK()[0]
+        assert np.array_equal(w, w.T)
+        e = _rel(w, ora.compute_wK([C])[0])
+        print(f"  ({a}, {b}) symmetric pair combined wK rel err {e:.2e}", flush=True)
+        assert e < 1e-11
+    # switching the combination off restores the plain attenuated tensor
+    jk.set_wcombine(False)
+    assert _rel(jk.get_Bw_dense(0, naux), Bw) < 1e-14

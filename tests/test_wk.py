@@ -130,3 +130,44 @@ def test_oracle_wk_near_exact_four_index():
     # and far from the Coulomb exchange: the attenuation is really applied
     K = o.compute([C])[2][0]
     assert np.abs(K - exact).max() > 0.1 * np.abs(exact).max()
+
+
+def test_oracle_wk_symmetric_pair_is_hermitivitized():
+    """lr_symmetric (no C_right): MemDFJK::compute_JK hermitivitizes wK (MemDFJK.cc:101-107);
+    with C_right given explicitly (= C_left) the same product is returned as built, and that
+    one is not symmetric, so the switch is visible."""
+    prim, aux, o = _water_oracle(OMEGA, "cholesky")
+    rng = np.random.default_rng(8)
+    C = np.linalg.qr(rng.standard_normal((prim.nbf(), prim.nbf())))[0][:, :5]
+    sym = o.compute_wK([C])[0]
+    raw = o.compute_wK([C], [C])[0]
+    assert np.abs(raw - raw.T).max() > 1e-6 * np.abs(raw).max()
+    assert np.array_equal(sym, sym.T)
+    assert np.abs(sym - 0.5 * (raw + raw.T)).max() < 1e-15 * np.abs(raw).max()
+
+
+def test_oracle_combined_exchange():
+    """wcombine (dfhelper.cc:659-667, 1411, 3022): K is not built and wK pairs J^-1 (Q|mn) C_l
+    with (alpha (Q|mn) + beta (Q|w|mn)) C_r.  (alpha, beta) = (1, 0) must give the plain K of
+    the J^-1/2 route (different algebra: the raw integrals against the full inverse metric),
+    (0, 1) the non-combined wK, and a general pair their combination."""
+    mol = M.water_zmat(0.96, 104.5)
+    prim = BasisSet.build(mol, "cc-pvdz")
+    aux = BasisSet.build(mol, "cc-pvdz-jkfit")
+    rng = np.random.default_rng(9)
+    n = prim.nbf()
+    Cl, Cr = rng.standard_normal((n, 4)), rng.standard_normal((n, 4))
+    plain = MemDFJKOracle(prim, aux, omega=OMEGA)
+    K = plain.compute([Cl], [Cr])[2][0]
+    wK = plain.compute_wK([Cl], [Cr])[0]
+    c10 = MemDFJKOracle(prim, aux, omega=OMEGA, wcombine=True, omega_alpha=1.0, omega_beta=0.0)
+    assert np.abs(c10.compute_wK([Cl], [Cr])[0] - K).max() < 1e-10 * np.abs(K).max()
+    assert not c10.compute([Cl], [Cr])[2][0].any()                 # K not built
+    J = plain.compute([Cl], [Cr])[1][0]
+    assert np.abs(c10.compute([Cl], [Cr])[1][0] - J).max() < 1e-13 * np.abs(J).max()
+    c01 = MemDFJKOracle(prim, aux, omega=OMEGA, wcombine=True, omega_alpha=0.0, omega_beta=1.0)
+    assert np.abs(c01.compute_wK([Cl], [Cr])[0] - wK).max() < 1e-13 * np.abs(wK).max()
+    a, b = 0.75, -0.4
+    cab = MemDFJKOracle(prim, aux, omega=OMEGA, wcombine=True, omega_alpha=a, omega_beta=b)
+    ref = a * K + b * wK
+    assert np.abs(cab.compute_wK([Cl], [Cr])[0] - ref).max() < 1e-10 * np.abs(ref).max()
