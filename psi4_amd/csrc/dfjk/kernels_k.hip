@@ -444,18 +444,21 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const uint16_t* Cl = Ch + C_SZ;
         const int fc = lane >> 4;
         bf8_t ah[4], al[4], bh[NBW], bl[NBW];
+        // hi fragments first; the lo fragments are read while the hi x hi MFMAs run
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            const int o = kg_swz(wm * 64 + i * 16 + (lane & 15), fc);
-            ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ah + o));
-            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + o));
-        }
+        for (int i = 0; i < NI; ++i)
+            ah[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ah + kg_swz(wm * 64 + i * 16 + (lane & 15), fc)));
 #pragma unroll
-        for (int j = 0; j < NBW; ++j) {
-            const int o = kg_swz(wn * NBW * 16 + j * 16 + (lane & 15), fc);
-            bh[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ch + o));
-            bl[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Cl + o));
-        }
+        for (int j = 0; j < NBW; ++j)
+            bh[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Ch + kg_swz(wn * NBW * 16 + j * 16 + (lane & 15), fc)));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < NBW; ++j)
+            bl[j] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Cl + kg_swz(wn * NBW * 16 + j * 16 + (lane & 15), fc)));
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+            al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + kg_swz(wm * 64 + i * 16 + (lane & 15), fc)));
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
