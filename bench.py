@@ -106,6 +106,7 @@ def scf_workload(args):
     del Bd, dB
     out = {"workload": f"RHF (H2O)_{nx * ny * nz} cc-pVDZ/cc-pVDZ-jkfit, seeded cluster",
            "nbf": nbf, "naux": naux, "nocc": int(mixed.nocc), "setup_s": round(t_setup, 2),
+           "setup_ms_by_phase": {k: round(v, 1) for k, v in info["setup_ms"].items()},
            "plain_fp64": {"E": rp.energy, "iterations": rp.iterations, "wall_s": round(t_plain, 3),
                           "jk_s": round(rp.timings["jk_s"], 3),
                           "jk_ms_per_iteration": [round(h["jk_s"] * 1e3, 2) for h in rp.history]},

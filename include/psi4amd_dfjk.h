@@ -87,6 +87,11 @@ int dfjk_set_omega(dfjk_handle* h, double omega);          /* JK::set_omega (jk.
  * omega_alpha * K + omega_beta * wK(omega) (dfhelper.cc:659-667, 1411, 3378-3438).  Effective at
  * the next wK compute; a tensor-level handle must set the attenuated tensor after it (and after B). */
 int dfjk_set_wcombine(dfjk_handle* h, int wcombine);
+/* Wall-clock ms of the set-up phases of the last dfjk_initialize, the counterpart of DFHelper's
+ * timers (dfhelper.cc:659-690, "DFH: AO Construction", "DFH: AO-Met. Contraction", ...):
+ * ms[0] Schwarz screening and tiles, [1] metric (P|Q), [2] metric factor, [3] (P|mn),
+ * [4] fit, [5] attenuated tensor (wK); the first min(n, 6) are written. */
+int dfjk_get_setup_timings(dfjk_handle* h, double* ms, int n);
 int dfjk_set_omega_alpha(dfjk_handle* h, double alpha);   /* JK::set_omega_alpha (jk.h:478-481), default 1 */
 int dfjk_set_omega_beta(dfjk_handle* h, double beta);     /* JK::set_omega_beta (jk.h:485-488), default 0 */
 int dfjk_set_precision(dfjk_handle* h, int precision);     /* enum dfjk_precision (no psi4 counterpart) */

@@ -190,6 +190,10 @@ struct dfjk_handle {
     // returns alpha K + beta wK; bw_* record what dBw holds
     bool wcombine = false;
     double omega_alpha = 1.0, omega_beta = 0.0;
+    // wall-clock ms of the set-up phases of the last initialize (psi4's DFH timers):
+    // screening + tiles, metric (P|Q), metric factor, (P|mn), fit, attenuated tensor
+    double setup_ms[6] = {0, 0, 0, 0, 0, 0};
+    bool dM_ready = false;  // ws_G holds the metric factor M (left by the device factorisation)
     bool bw_comb = false;
     double bw_alpha = 1.0, bw_beta = 0.0;
     std::vector<double> fit_M;      // host metric factor, kept for rebuilding Bw after an omega change
@@ -343,20 +347,116 @@ void prepare_sparsity(dfjk_handle* h) {
 }
 
 // Metric factor on the host (FP64): returns M with B = M (P|mn).
-std::vector<double> metric_factor(dfjk_handle* h) {
+// Wall-clock laps of the set-up phases.
+struct Lap {
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    double operator()() {
+        const auto n = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(n - t).count();
+        t = n;
+        return ms;
+    }
+};
+
+[[noreturn]] void cholesky_failed(int pivot) {
+    fail(DFJK_ERR_ARG, "metric Cholesky failed at pivot " + std::to_string(pivot) +
+                           " (metric not positive definite; use DFJK_FIT_EIGEN)");
+}
+
+// M = L^-1 with J = L L^T, on the device.  Right-looking blocked Cholesky: each NB x NB
+// diagonal block is factorised and inverted on the host (cholesky_lower, lower_inverse), the
+// panel below it becomes L_ik = A_ik L_kk^-T and the trailing block A_22 -= L_i L_i^T (FP64
+// MFMA GEMMs); then X = L^-1 by block forward substitution, X_i = -L_ii^-1 (L_i,<i X_<i), with
+// the host inverses of the diagonal blocks.  M stays in ws_G for the fit (dM_ready); it is
+// returned on the host (row-major, lower) only when want_host.
+std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<double>& Jm, bool want_host) {
+    const int n = h->naux;
+    constexpr int NB = 256;
+    const hipStream_t s = h->stream;
+    const size_t nn = static_cast<size_t>(n) * n;
+    DevBuf dA, dP, dPT, dT, dL;
+    h->dM_ready = false;
+    double* A = dA.get<double>(nn);
+    double* X = h->ws_G.get<double>(nn);
+    double* P = dP.get<double>(static_cast<size_t>(n) * NB);
+    double* PT = dPT.get<double>(static_cast<size_t>(n) * NB);
+    double* T = dT.get<double>(static_cast<size_t>(n) * NB);
+    double* Li = dL.get<double>(static_cast<size_t>(NB) * NB);
+    const size_t ld = static_cast<size_t>(n) * sizeof(double);
+    hcheck(hipMemcpyAsync(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload metric");
+    hcheck(hipMemsetAsync(X, 0, nn * sizeof(double), s), "memset");
+    std::vector<double> blk(NB * NB), invT(NB * NB);
+    std::vector<std::vector<double>> dinv;  // host inverses of the diagonal blocks
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int nb = std::min(NB, n - k0), k1 = k0 + nb, m = n - k1;
+        const size_t bw = static_cast<size_t>(nb) * sizeof(double);
+        double* Akk = A + static_cast<size_t>(k0) * n + k0;
+        hcheck(hipMemcpy2DAsync(blk.data(), bw, Akk, ld, bw, nb, hipMemcpyDeviceToHost, s), "diagonal block");
+        hcheck(hipStreamSynchronize(s), "diagonal block");
+        const int bad = cholesky_lower(blk.data(), nb, h->nthreads);
+        if (bad >= 0) cholesky_failed(k0 + bad);
+        dinv.emplace_back(static_cast<size_t>(nb) * nb);
+        std::vector<double>& inv = dinv.back();
+        lower_inverse(blk.data(), nb, inv.data(), h->nthreads);
+        for (int r = 0; r < nb; ++r)
+            for (int c = 0; c < nb; ++c) invT[static_cast<size_t>(c) * nb + r] = inv[static_cast<size_t>(r) * nb + c];
+        hcheck(hipMemcpy2DAsync(Akk, ld, blk.data(), bw, bw, nb, hipMemcpyHostToDevice, s), "L_kk");
+        hcheck(hipMemcpy2DAsync(X + static_cast<size_t>(k0) * n + k0, ld, inv.data(), bw, bw, nb,
+                                hipMemcpyHostToDevice, s),
+               "X_kk");
+        if (m > 0) {
+            hcheck(hipMemcpyAsync(Li, invT.data(), bw * nb, hipMemcpyHostToDevice, s), "L_kk^-T");
+            double* Aik = A + static_cast<size_t>(k1) * n + k0;
+            launch_gemm_nn_f64(Aik, n, Li, nb, P, nb, m, nb, nb, 1.0, 0.0, s);
+            hcheck(hipMemcpy2DAsync(Aik, ld, P, bw, bw, m, hipMemcpyDeviceToDevice, s), "panel");
+            launch_transpose(P, m, nb, nb, PT, m, s);
+            launch_gemm_nn_f64(P, nb, PT, m, A + static_cast<size_t>(k1) * n + k1, n, m, m, nb, -1.0, 1.0, s);
+        }
+        // the host blocks are rewritten in the next pass: the copies above must have run
+        hcheck(hipStreamSynchronize(s), "panel");
+    }
+    for (int i0 = NB, bi = 1; i0 < n; i0 += NB, ++bi) {
+        const int nb = std::min(NB, n - i0);
+        launch_gemm_nn_f64(A + static_cast<size_t>(i0) * n, n, X, n, T, i0, nb, i0, i0, 1.0, 0.0, s);
+        hcheck(hipMemcpyAsync(Li, dinv[bi].data(), static_cast<size_t>(nb) * nb * sizeof(double),
+                              hipMemcpyHostToDevice, s),
+               "L_ii^-1");
+        launch_gemm_nn_f64(Li, nb, T, i0, X + static_cast<size_t>(i0) * n, n, nb, i0, nb, -1.0, 0.0, s, 1, 0, 0, 0,
+                           1);
+        hcheck(hipStreamSynchronize(s), "block row");
+    }
+    h->dM_ready = true;
+    std::vector<double> M;
+    if (want_host) {
+        M.resize(nn);
+        hcheck(hipMemcpyAsync(M.data(), X, nn * sizeof(double), hipMemcpyDeviceToHost, s), "download M");
+        hcheck(hipStreamSynchronize(s), "download M");
+    }
+    return M;
+}
+
+// The metric factor M = L^-1; on the host only when want_host (else it is left in ws_G).
+std::vector<double> metric_factor(dfjk_handle* h, bool want_host) {
     const int na = h->naux;
+    Lap lap;
     std::vector<double> Jm(static_cast<size_t>(na) * na);
     metric(h->aux, Jm.data(), h->nthreads);
-    std::vector<double> M(static_cast<size_t>(na) * na);
+    h->setup_ms[1] = lap();
+    std::vector<double> M;
     if (h->fitting == DFJK_FIT_CHOLESKY) {
-        const int bad = cholesky_lower(Jm.data(), na, h->nthreads);
-        if (bad >= 0)
-            fail(DFJK_ERR_ARG, "metric Cholesky failed at pivot " + std::to_string(bad) +
-                                   " (metric not positive definite; use DFJK_FIT_EIGEN)");
-        lower_inverse(Jm.data(), na, M.data(), h->nthreads);
+        if (std::getenv("PSI4AMD_HOST_FACTOR")) {
+            h->dM_ready = false;
+            M.resize(static_cast<size_t>(na) * na);
+            const int bad = cholesky_lower(Jm.data(), na, h->nthreads);
+            if (bad >= 0) cholesky_failed(bad);
+            lower_inverse(Jm.data(), na, M.data(), h->nthreads);
+        } else {
+            M = device_metric_factor(h, Jm, want_host);
+        }
     } else {
         fail(DFJK_ERR_ARG, "DFJK_FIT_EIGEN is not available in this build");
     }
+    h->setup_ms[2] = lap();
     return M;
 }
 
@@ -367,7 +467,8 @@ void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vect
     const int na = h->naux;
     const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
     double* dM = h->ws_G.get<double>(static_cast<size_t>(na) * na);
-    hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
+    if (!h->dM_ready)
+        hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
     // chunk of columns so that the staging buffer stays <= ~1 GiB
     size_t cw = std::max<size_t>(TILE2, (size_t(1) << 27) / std::max(1, na));
     cw = std::min(ncolB, cw / TILE2 * TILE2);
@@ -380,19 +481,20 @@ void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vect
                "upload (P|mn)");
         if (host_dst) {
             launch_gemm_nn_f64(dM, na, dA, static_cast<int>(w), dBp, static_cast<int>(w), na, static_cast<int>(w), na,
-                               1.0, 0.0, h->stream);
+                               1.0, 0.0, h->stream, 1, 0, 0, 0, 1);
             hcheck(hipMemcpy2DAsync(host_dst + c0, ncolB * sizeof(double), dBp, w * sizeof(double),
                                     w * sizeof(double), na, hipMemcpyDeviceToHost, h->stream),
                    "download fitted B");
         } else {
             launch_gemm_nn_f64(dM, na, dA, static_cast<int>(w), dBp + c0, static_cast<int>(ncolB), na,
-                               static_cast<int>(w), na, 1.0, 0.0, h->stream);
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, 1);
         }
         hcheck(hipStreamSynchronize(h->stream), "fit");
     }
     h->ws_XL.release();
     h->ws_XR.release();
     h->ws_G.release();
+    h->dM_ready = false;
 }
 
 // ---------------------------------------------------------------- device (P|mn)
@@ -535,7 +637,8 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
     const int na = h->naux;
     const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
     double* dM = h->ws_G.get<double>(static_cast<size_t>(na) * na);
-    hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
+    if (!h->dM_ready)
+        hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
     if (!host_dst) {
         double* dBp = dst.get<double>(b_doubles(h));
         size_t cw = std::max<size_t>(TILE2, (size_t(1) << 30) / (sizeof(double) * std::max(1, na)));
@@ -543,7 +646,7 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
         for (size_t c0 = 0; c0 < ncolB; c0 += cw) {
             const size_t w = std::min(cw, ncolB - c0);
             launch_gemm_nn_f64(dM, na, A + c0, static_cast<int>(ncolB), dBp + c0, static_cast<int>(ncolB), na,
-                               static_cast<int>(w), na, 1.0, 0.0, h->stream);
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, 1);
         }
     } else {
         size_t cw = std::max<size_t>(TILE2, (size_t(1) << 27) / std::max(1, na));
@@ -552,7 +655,7 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
         for (size_t c0 = 0; c0 < ncolB; c0 += cw) {
             const size_t w = std::min(cw, ncolB - c0);
             launch_gemm_nn_f64(dM, na, A + c0, static_cast<int>(ncolB), tmp, static_cast<int>(w), na,
-                               static_cast<int>(w), na, 1.0, 0.0, h->stream);
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, 1);
             hcheck(hipMemcpy2DAsync(host_dst + c0, ncolB * sizeof(double), tmp, w * sizeof(double),
                                     w * sizeof(double), na, hipMemcpyDeviceToHost, h->stream),
                    "download fitted B");
@@ -562,6 +665,7 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
     }
     hcheck(hipStreamSynchronize(h->stream), "fit");
     h->ws_G.release();
+    h->dM_ready = false;
 }
 
 // Resident doubles of the in-core algorithm (B, Bw when wK is on, the metric).  AUTO decides
@@ -639,7 +743,7 @@ void combine_Bw(dfjk_handle* h) {
 void build_Bw(dfjk_handle* h) {
     if (h->tensor_mode) fail(DFJK_ERR_STATE, "wK needs the attenuated tensor: dfjk_set_Bw_dense");
     if (!(h->omega > 0.0)) fail(DFJK_ERR_ARG, "wK needs omega > 0 (dfjk_set_omega)");
-    if (h->fit_M.empty()) h->fit_M = metric_factor(h);
+    if (h->fit_M.empty()) h->fit_M = metric_factor(h, true);
     if (gpu_three_index(h, h->omega, h->ws_XL)) {
         fit_device_A(h, h->fit_M, h->ws_XL.as<double>(), h->dBw);
         h->ws_XL.release();
@@ -656,16 +760,21 @@ void build_Bw(dfjk_handle* h) {
 }
 
 void initialize_basis(dfjk_handle* h) {
+    for (double& t : h->setup_ms) t = 0.0;
+    Lap lap;
     if (h->tiles.empty()) {
         prepare_sparsity(h);
         build_tiles(h);
     }
     upload_tiles(h);
-    std::vector<double> M = metric_factor(h);
+    h->setup_ms[0] = lap();
+    std::vector<double> M = metric_factor(h, h->do_wK);  // the host copy rebuilds Bw after an omega change
     choose_subalgorithm(h);
+    lap();
     // (P|mn) directly in the tiled layout: on the device when the classes and memory allow,
     // else on the host
     if (gpu_three_index(h, 0.0, h->ws_XL)) {
+        h->setup_ms[3] = lap();
         fit_device_A(h, M, h->ws_XL.as<double>(), h->dB, h->ooc ? h->hB : nullptr);
         h->ws_XL.release();
     } else {
@@ -674,11 +783,15 @@ void initialize_basis(dfjk_handle* h) {
         std::vector<double> A(static_cast<size_t>(h->naux) * ncolB, 0.0);
         three_index(h->prim, h->aux, dst.data(), static_cast<int64_t>(ncolB), h->shell_mask.data(), A.data(),
                     h->nthreads);
+        h->setup_ms[3] = lap();
         fit_on_device(h, M, A, h->dB, h->ooc ? h->hB : nullptr);
     }
+    hcheck(hipStreamSynchronize(h->stream), "fit");
+    h->setup_ms[4] = lap();
     if (h->do_wK) {
         h->fit_M = std::move(M);
         build_Bw(h);
+        h->setup_ms[5] = lap();
     }
 }
 
@@ -1358,6 +1471,7 @@ void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double*
     const double* dvec = h->ws_d.as<double>();
 
     // G = 2J - K; E = <D, G> (fixed-order reduction); dE/dC = 4 G C
+    h->dM_ready = false;
     double* G = h->ws_G.get<double>(nn);
     launch_axpby_mat(Jm, 2.0, Km, -1.0, G, nn, s);
     if (E) {
@@ -1530,6 +1644,13 @@ int dfjk_set_subalgorithm(dfjk_handle* h, int v) {
         if (v < DFJK_SUBALGO_AUTO || v > DFJK_SUBALGO_OUT_OF_CORE) fail(DFJK_ERR_ARG, "unknown subalgorithm");
         if (h->initialized) fail(DFJK_ERR_STATE, "the subalgorithm must be chosen before initialize()");
         h->subalgo = v;
+    });
+}
+int dfjk_get_setup_timings(dfjk_handle* h, double* ms, int n) {
+    return guarded([&] {
+        require_state(h, false);
+        if (n < 0 || (n > 0 && !ms)) fail(DFJK_ERR_ARG, "bad timing buffer");
+        for (int i = 0; i < n && i < 6; ++i) ms[i] = h->setup_ms[i];
     });
 }
 int dfjk_set_wcombine(dfjk_handle* h, int wcombine) {
@@ -1829,6 +1950,7 @@ int dfjk_finalize(dfjk_handle* h) {
                         &h->ws_Y,    &h->ws_PC,  &h->ws_gam, &h->ws_dc, &h->ws_stage[0], &h->ws_stage[1],
                         &h->ws_stage32, &h->ws_stagebf};
         for (DevBuf* b : ws) b->release();
+        h->dM_ready = false;
     });
 }
 

@@ -230,46 +230,78 @@ __global__ void k_tile_reduce(const double* __restrict__ P, int nsplit, int ntil
     *o = alpha * s + (accumulate ? *o : 0.0);
 }
 
-// Row-major C = alpha*A*B + beta*C, 128x128 output tiles, FP64 MFMA.
+// Row-major C = alpha*A*B + beta*C, 128x128 output tiles, FP64 MFMA, 4 waves of 64x64.
 // grid = (ceil(N/128), ceil(M/128), batch); operand z is offset by z * stride (0 = shared).
-__global__ __launch_bounds__(256) void k_gemm_nn_f64(const double* __restrict__ A, int lda,
-                                                     const double* __restrict__ Bm, int ldb, double* __restrict__ Cm,
-                                                     int ldc, int M, int N, int K, double alpha, double beta,
-                                                     size_t sA, size_t sB, size_t sC) {
-    __shared__ double As[128][17];
-    __shared__ double Bs[16][132];
+// lower_a: A is lower triangular (A[m][k] = 0 for k > m), so row block m0 stops at k = m0 + 128.
+// K-step 16, double-buffered LDS: the next step's operands are loaded into registers before
+// the current step's MFMAs and stored to the other buffer after them, one barrier per step.
+// Out-of-range elements load as zero.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_nn_f64(
+    const double* __restrict__ A, int lda, const double* __restrict__ Bm, int ldb, double* __restrict__ Cm, int ldc,
+    int M, int N, int K, double alpha, double beta, size_t sA, size_t sB, size_t sC, int lower_a) {
+    constexpr int KS = 16;
+    __shared__ double As[2][128][17];
+    __shared__ double Bs[2][16][132];
     A += blockIdx.z * sA;
     Bm += blockIdx.z * sB;
     Cm += blockIdx.z * sC;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
+    const int kend = lower_a ? min(K, m0 + 128) : K;
+    const int nsteps = (kend + KS - 1) / KS;
     d4_t acc[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = d4_t{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        __syncthreads();
+    // thread slots: A element (r, kk) = (e >> 4, e & 15), B element (kb, cc) = (e >> 7, e & 127)
+    double ra[8], rb[8];
+    auto gload = [&](int st) {
+        const int k0 = st * KS;
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             const int e = tid + 256 * it;
             const int r = e >> 4, kk = e & 15;
-            As[r][kk] = (m0 + r < M && k0 + kk < K) ? A[static_cast<size_t>(m0 + r) * lda + k0 + kk] : 0.0;
+            ra[it] = (m0 + r < M && k0 + kk < K) ? A[static_cast<size_t>(m0 + r) * lda + k0 + kk] : 0.0;
             const int kb = e >> 7, cc = e & 127;
-            Bs[kb][cc] = (k0 + kb < K && n0 + cc < N) ? Bm[static_cast<size_t>(k0 + kb) * ldb + n0 + cc] : 0.0;
+            rb[it] = (k0 + kb < K && n0 + cc < N) ? Bm[static_cast<size_t>(k0 + kb) * ldb + n0 + cc] : 0.0;
         }
-        __syncthreads();
+    };
+    auto lstore = [&](int buf) {
 #pragma unroll
-        for (int ks = 0; ks < 16; ks += 4) {
+        for (int it = 0; it < 8; ++it) {
+            const int e = tid + 256 * it;
+            As[buf][e >> 4][e & 15] = ra[it];
+            Bs[buf][e >> 7][e & 127] = rb[it];
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ks += 4) {
             double a[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[wr * 64 + i * 16 + (lane & 15)][ks + (lane >> 4)];
+            for (int i = 0; i < 4; ++i) a[i] = As[buf][wr * 64 + i * 16 + (lane & 15)][ks + (lane >> 4)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = Bs[ks + (lane >> 4)][wc * 64 + j * 16 + (lane & 15)];
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[buf][ks + (lane >> 4)][wc * 64 + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     acc[i * 4 + j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i * 4 + j], 0, 0, 0);
+        }
+    };
+    if (nsteps > 0) {
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        for (int st = 0; st < nsteps; ++st) {
+            const bool more = st + 1 < nsteps;
+            if (more) gload(st + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(buf);
+            if (more) lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
         }
     }
 #pragma unroll
@@ -408,10 +440,13 @@ void launch_tile_reduce(const double* P, int nsplit, int ntile, const int* d_til
 }
 
 void launch_gemm_nn_f64(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
-                        double alpha, double beta, hipStream_t s, int batch, size_t sA, size_t sB, size_t sC) {
+                        double alpha, double beta, hipStream_t s, int batch, size_t sA, size_t sB, size_t sC,
+                        int lower_a) {
     require(lda >= K && ldb >= N && ldc >= N && batch >= 1 && batch <= 65535, "gemm_nn_f64");
+    if (M <= 0 || N <= 0) return;
     dim3 grid((N + 127) / 128, (M + 127) / 128, batch);
-    hipLaunchKernelGGL(k_gemm_nn_f64, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, alpha, beta, sA, sB, sC);
+    hipLaunchKernelGGL(k_gemm_nn_f64, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, alpha, beta, sA, sB, sC,
+                       lower_a);
 }
 
 // ---- adjoint helpers ----

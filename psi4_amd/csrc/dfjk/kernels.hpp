@@ -59,9 +59,10 @@ void launch_tile_reduce(const double* P, int nsplit, int ntile, const int* d_til
 
 // Plain FP64 GEMM, row-major, C = alpha*A*B + beta*C (A MxK lda, B KxN ldb).  MFMA, 128x128 tiles.
 // batch > 1: operand z at A + z*sA, B + z*sB, C + z*sC (a zero stride shares the operand).
+// lower_a: A lower triangular (its zero upper part is skipped)
 void launch_gemm_nn_f64(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
                         double alpha, double beta, hipStream_t s, int batch = 1, size_t sA = 0, size_t sB = 0,
-                        size_t sC = 0);
+                        size_t sC = 0, int lower_a = 0);
 
 // ---- adjoint helpers ----
 // Z = a X + b Y (n elements)

@@ -289,6 +289,12 @@ class JK {
         int ooc = 0;
         check(dfjk_get_out_of_core(h_, &ooc));
         d["out_of_core"] = ooc != 0;
+        double ms[6];
+        check(dfjk_get_setup_timings(h_, ms, 6));
+        py::dict t;
+        const char* names[6] = {"screening", "metric", "metric_factor", "three_index", "fit", "attenuated"};
+        for (int i = 0; i < 6; ++i) t[names[i]] = ms[i];
+        d["setup_ms"] = t;
         return d;
     }
 

@@ -1,5 +1,9 @@
 """Three-index integrals (P|mn) on the device against the host engine.
 
+The metric factor L^-1 (J = L L^T) from the device's blocked Cholesky and block forward
+substitution gives the same fitted tensor as the host factorisation ((H2O)_16, naux 1856 in
+256-row blocks with a partial last block; 1e-11 of the largest element).
+
 The fitted tensor B = L^-1 (P|mn) built from the device integrals equals the one built from
 the host engine's integrals (same device fit) to 1e-12 of the largest element, for water and
 (H2O)_8 in cc-pVDZ / cc-pVDZ-jkfit (s..d | s..f) and water in cc-pVTZ / cc-pVTZ-jkfit
@@ -18,12 +22,11 @@ from conftest import announce
 pytestmark = pytest.mark.gpu
 
 
-def _tensor(mol, basis, device, omega=None):
+def _tensor(mol, basis, device, omega=None, key="PSI4AMD_HOST_INTS", info=None):
     from psi4_amd.basisset import BasisSet
     from psi4_amd.core import JK
     prim = BasisSet.build(mol, basis)
     aux = BasisSet.build(mol, basis + "-jkfit")
-    key = "PSI4AMD_HOST_INTS"
     old = os.environ.pop(key, None)
     if not device:
         os.environ[key] = "1"
@@ -36,6 +39,8 @@ def _tensor(mol, basis, device, omega=None):
         jk.initialize()
         dt = time.perf_counter() - t0
         naux = jk.info()["naux"]
+        if info is not None:
+            info.update(jk.info())
         B = jk.get_B_dense(0, naux)
         Bw = jk.get_Bw_dense(0, naux) if omega else None
     finally:
@@ -71,3 +76,18 @@ def test_device_three_index_attenuated(gpu_available):
     Bh, Bwh, _ = _tensor(mol, "cc-pvdz", False, omega=0.3)
     print(f"  B {_rel(Bd, Bh):.2e}  Bw {_rel(Bwd, Bwh):.2e}", flush=True)
     assert _rel(Bd, Bh) < 1e-12 and _rel(Bwd, Bwh) < 1e-12
+
+
+def test_device_metric_factor_matches_host(gpu_available):
+    mol = M.water_cluster((4, 2, 2), seed=4)
+    announce("metric factor (H2O)_16 cc-pVDZ-jkfit: device blocked Cholesky vs host")
+    info = {}
+    Bd, _, td = _tensor(mol, "cc-pvdz", True, key="PSI4AMD_HOST_FACTOR", info=info)
+    Bh, _, th = _tensor(mol, "cc-pvdz", False, key="PSI4AMD_HOST_FACTOR")
+    e = _rel(Bd, Bh)
+    t = info["setup_ms"]
+    print(f"  naux {info['naux']}: |B_dev - B_host|/max {e:.2e}; initialize {td:.2f} s device factor, "
+          f"{th:.2f} s host factor; phases (ms) {t}", flush=True)
+    assert info["naux"] == 1856
+    assert e < 1e-11
+    assert t["metric"] > 0 and t["metric_factor"] > 0 and t["fit"] > 0 and t["attenuated"] == 0
