@@ -90,7 +90,7 @@ class RHF:
     """Closed-shell SCF with a DF JK object (MEM_DF path)."""
 
     def __init__(self, mol, basis="cc-pvdz", df_basis_scf=None, jk=None, e_convergence=1e-10, d_convergence=1e-8,
-                 maxiter=100, diis_max_vecs=10, mixed_precision=None, mixed_until=1e-5, incfock_full_every=5,
+                 maxiter=100, diis_max_vecs=10, mixed_precision=None, mixed_until=1e-7, incfock_full_every=8,
                  fail_on_maxiter=True, jk_callback=None, perturb_dipole=None):
         from .basisset import BasisSet
         self.mol = mol
