@@ -1197,17 +1197,17 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                 if (prec == DFJK_PREC_FP64) {
                     double* xl = reinterpret_cast<double*>(XL) + pp.offl;
                     double* xr = reinterpret_cast<double*>(XR) + pp.offr;
-                    launch_kgemm2_f64(xl, xr, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha,
+                    launch_kgemm2_f64(xl, xr, xq, xrows, nbf, ldc, pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha,
                                       acc, upper ? 1 : 0, s);
                 } else if (prec == DFJK_PREC_FP32) {
                     float* xl = reinterpret_cast<float*>(XL) + pp.offl;
                     float* xr = reinterpret_cast<float*>(XR) + pp.offr;
-                    launch_kgemm2_f32(xl, xr, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha,
+                    launch_kgemm2_f32(xl, xr, xq, xrows, nbf, ldc, pp.kbeg, pp.kend, nq, nsp, otiles, ntile, P, pp.alpha,
                                       acc, upper ? 1 : 0, s);
                 } else {
                     uint16_t* xl = reinterpret_cast<uint16_t*>(XL) + pp.offl;
                     uint16_t* xr = reinterpret_cast<uint16_t*>(XR) + pp.offr;
-                    launch_kgemm_bf3(xl, xl + xq * qblk, xr, xr + xq * qblk, xq, xrows, ldc, pp.kbeg, pp.kend, nq, nsp,
+                    launch_kgemm_bf3(xl, xl + xq * qblk, xr, xr + xq * qblk, xq, xrows, nbf, ldc, pp.kbeg, pp.kend, nq, nsp,
                                      otiles, ntile, P, pp.alpha, acc, upper ? 1 : 0, s);
                 }
                 h->n_kgemm += 1;
@@ -1278,7 +1278,7 @@ void compute_wk_pair(dfjk_handle* h, const double* dCl, const double* dCr, int n
         tm.end(T_HT, e0);
         h->n_ht += 2;
         hipEvent_t e1 = tm.begin();
-        launch_kgemm2_f64(XL, XR, xq, xrows, ldc, 0, kend, nq, nsp, otiles, ntile, P, 1.0, blk > 0 ? 1 : 0, 0, s);
+        launch_kgemm2_f64(XL, XR, xq, xrows, nbf, ldc, 0, kend, nq, nsp, otiles, ntile, P, 1.0, blk > 0 ? 1 : 0, 0, s);
         h->n_kgemm += 1;
         if (nsp < nsplit && blk == 0)
             hcheck(hipMemsetAsync(P + static_cast<size_t>(nsp) * ntile * 16384, 0,

@@ -536,7 +536,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                                                     size_t strideB, int ld, int kbeg, int kend, int nbatch,
                                                     int nsplit, const int2* __restrict__ tiles, int ntile,
                                                     double* __restrict__ P, double alpha, int accumulate,
-                                                    int upper) {
+                                                    int upper, int nvalid) {
     constexpr int KS = 16, AP = KS + 1;
     constexpr bool F64 = sizeof(T) == 8;
     typedef typename std::conditional<F64, d4_t, f4_t>::type acc_t;
@@ -548,9 +548,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (!xi.valid) return;
     const int tile = xi.item, split = xi.group;
     const int2 tb = tiles[tile];
-    // symmetric result: only the upper triangle of a diagonal tile is read, so the wave
-    // whose 64x64 block lies below it does the staging but no MFMAs
-    const bool idle = upper && tb.x == tb.y && wr > wc;
+    // a wave does the staging but no MFMAs when its 64x64 block lies below the diagonal of a
+    // diagonal tile of a symmetric result (only the upper triangle is read) or entirely in the
+    // zero padding rows / columns of X past nvalid
+    const bool idle = (upper && tb.x == tb.y && wr > wc) || tb.x * 128 + wr * 64 >= nvalid ||
+                      tb.y * 128 + wc * 64 >= nvalid;
     const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
     const int b1 = static_cast<int>((static_cast<long long>(nbatch) * (split + 1)) / nsplit);
     const int nks = (kend - kbeg) / KS;
@@ -717,7 +719,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_bf3(
     const uint16_t* __restrict__ Ah_, const uint16_t* __restrict__ Al_, const uint16_t* __restrict__ Bh_,
     const uint16_t* __restrict__ Bl_, size_t stride, int ld, int kbeg, int kend, int nbatch, int nsplit,
-    const int2* __restrict__ tiles, int ntile, double* __restrict__ P, double alpha, int accumulate, int upper) {
+    const int2* __restrict__ tiles, int ntile, double* __restrict__ P, double alpha, int accumulate, int upper,
+    int nvalid) {
     constexpr int KS = 32, SZ = 128 * 32;
     __shared__ uint16_t Ls[2][4 * SZ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -726,7 +729,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (!xi.valid) return;
     const int tile = xi.item, split = xi.group;
     const int2 tb = tiles[tile];
-    const bool idle = upper && tb.x == tb.y && wr > wc;  // below the diagonal of a diagonal tile
+    // no MFMAs for the wave below the diagonal of a diagonal tile of a symmetric result, or
+    // for one entirely in the zero padding rows / columns of X past nvalid
+    const bool idle = (upper && tb.x == tb.y && wr > wc) || tb.x * 128 + wr * 64 >= nvalid ||
+                      tb.y * 128 + wc * 64 >= nvalid;
     const int m0 = tb.x * 128, n0 = tb.y * 128;
     const int b0 = static_cast<int>((static_cast<long long>(nbatch) * split) / nsplit);
     const int b1 = static_cast<int>((static_cast<long long>(nbatch) * (split + 1)) / nsplit);
@@ -1028,26 +1034,29 @@ static void kgemm_check(int xrows, int ld, int kbeg, int kend, int nbatch, int n
             "kgemm shape");
 }
 
-void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrows, int ld, int kbeg, int kend,
+void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrows, int nvalid, int ld, int kbeg,
+                       int kend, int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha,
+                       int accumulate, int upper, hipStream_t s) {
+    kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
+    require(nvalid > 0 && nvalid <= xrows, "kgemm valid rows");
+    hipLaunchKernelGGL((k_kgemm_mfma<double>), dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, A, stride, B, stride, ld,
+                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper, nvalid);
+}
+void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int nvalid, int ld, int kbeg, int kend,
                        int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
                        int upper, hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
-    hipLaunchKernelGGL((k_kgemm_mfma<double>), dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, A, stride, B, stride, ld,
-                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper);
-}
-void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int ld, int kbeg, int kend, int nbatch,
-                       int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate, int upper,
-                       hipStream_t s) {
-    kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 16);
+    require(nvalid > 0 && nvalid <= xrows, "kgemm valid rows");
     hipLaunchKernelGGL((k_kgemm_mfma<float>), dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, A, stride, B, stride, ld,
-                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper);
+                       kbeg, kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper, nvalid);
 }
 void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh, const uint16_t* Bl, size_t stride,
-                      int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles, int ntile,
-                      double* P, double alpha, int accumulate, int upper, hipStream_t s) {
+                      int xrows, int nvalid, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles,
+                      int ntile, double* P, double alpha, int accumulate, int upper, hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 32);
+    require(nvalid > 0 && nvalid <= xrows, "kgemm valid rows");
     hipLaunchKernelGGL(k_kgemm_bf3, dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, Ah, Al, Bh, Bl, stride, ld, kbeg,
-                       kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper);
+                       kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper, nvalid);
 }
 
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s) {

@@ -29,15 +29,16 @@ void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, co
 
 // P[split][tile] (+)= alpha * sum_b A_b[:, kbeg:kend] B_b[:, kbeg:kend]^T over 128x128 tiles;
 // A_b = A + b*stride, rows of length ld, xrows rows (a multiple of 128).
-void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrows, int ld, int kbeg, int kend,
+// nvalid: rows of X that are not zero padding (nbf); fragments past it are skipped
+void launch_kgemm2_f64(const double* A, const double* B, size_t stride, int xrows, int nvalid, int ld, int kbeg,
+                       int kend, int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha,
+                       int accumulate, int upper, hipStream_t s);
+void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int nvalid, int ld, int kbeg, int kend,
                        int nbatch, int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
                        int upper, hipStream_t s);
-void launch_kgemm2_f32(const float* A, const float* B, size_t stride, int xrows, int ld, int kbeg, int kend, int nbatch,
-                       int nsplit, const int2* tiles, int ntile, double* P, double alpha, int accumulate,
-                       int upper, hipStream_t s);
 void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh, const uint16_t* Bl, size_t stride,
-                      int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles, int ntile,
-                      double* P, double alpha, int accumulate, int upper, hipStream_t s);
+                      int xrows, int nvalid, int ld, int kbeg, int kend, int nbatch, int nsplit, const int2* tiles,
+                      int ntile, double* P, double alpha, int accumulate, int upper, hipStream_t s);
 
 // Precision copies and operand packing (cmap: destination column -> source column or -1).
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
