@@ -5,12 +5,13 @@ form_F; compute_E = E_nuc + 1/2 tr((D_a + D_b) H) + 1/2 sum_s tr(D_s F_s); orbit
 gradient X^T (F_s D_s S - S D_s F_s) X of both spins, concatenated for the Pulay
 DIIS of procrouting/diis.py) on the in-core DF JK: each iteration is one
 ``compute()`` with C_left = [C_a,occ, C_b,occ] (lr_symmetric), i.e. two density
-pairs through the engine at once.
+pairs through the engine at once.  As in the RHF driver, the SCF algebra (orthogonalised
+eigensolves, DIIS, orbital gradient) runs in torch FP64 on the engine's device.
 """
 from __future__ import annotations
 
 import numpy as np
-from scipy.linalg import eigh
+import torch
 
 from . import ints
 from .core import JK, PREC_FP64
@@ -48,30 +49,52 @@ class UHF:
             self._jk.initialize()
         return self._jk
 
+    def _device(self):
+        """As RHF: the SCF algebra runs on the GPU next to the device engine, else on the host."""
+        jk = self.jk()
+        if hasattr(jk, "compute_device") and torch.cuda.is_available():
+            return torch.device("cuda", torch.cuda.current_device())
+        return torch.device("cpu")
+
     def _build(self, Ca_occ, Cb_occ):
+        """J_a, J_b, K_a, K_b of the two occupied sets, one compute() with two pairs."""
         jk = self.jk()
         jk.set_precision(PREC_FP64)
+        if Ca_occ.is_cuda:
+            n = Ca_occ.shape[0]
+            A, B = Ca_occ.contiguous(), Cb_occ.contiguous()
+            out = [torch.empty((n, n), dtype=torch.float64, device=A.device) for _ in range(4)]
+            torch.cuda.synchronize()           # C ready before the engine's stream reads it
+            jk.compute_device([A.data_ptr(), B.data_ptr()], [], [A.shape[1], B.shape[1]], [],
+                              [out[0].data_ptr(), out[1].data_ptr()], [out[2].data_ptr(), out[3].data_ptr()], False)
+            torch.cuda.synchronize()
+            return out
         jk.C_clear()
-        jk.C_left_add(np.ascontiguousarray(Ca_occ))
-        jk.C_left_add(np.ascontiguousarray(Cb_occ))
+        jk.C_left_add(np.ascontiguousarray(Ca_occ.numpy()))
+        jk.C_left_add(np.ascontiguousarray(Cb_occ.numpy()))
         jk.compute()
         J, K = jk.J(), jk.K()
-        return np.array(J[0]), np.array(J[1]), np.array(K[0]), np.array(K[1])
+        return [torch.from_numpy(np.array(x)) for x in (J[0], J[1], K[0], K[1])]
 
     def compute_energy(self):
         mol, bs = self.mol, self.basis
-        S = ints.overlap(bs)
-        H = ints.kinetic(bs) + ints.potential(bs, mol)
+        dev = self._device()
+        S = torch.from_numpy(ints.overlap(bs)).to(dev)
+        H = torch.from_numpy(ints.kinetic(bs) + ints.potential(bs, mol)).to(dev)
         enuc = mol.nuclear_repulsion_energy()
-        w, U = np.linalg.eigh(S)
-        X = (U * w ** -0.5) @ U.T
-        self.jk()
+        w, U = torch.linalg.eigh(S)
+        X = (U * w.rsqrt()) @ U.T
+
+        def diag(F):
+            e, Cp = torch.linalg.eigh(X.T @ F @ X)
+            return e, X @ Cp
+
         if self.guess is None:
-            eps_a, Ca = eigh(H, S)
+            eps_a, Ca = diag(H)
         else:
-            Ca = np.array(self.guess, dtype=np.float64)
-            eps_a = np.zeros(Ca.shape[1])
-        eps_b, Cb = eps_a.copy(), Ca.copy()
+            Ca = torch.as_tensor(np.array(self.guess, dtype=np.float64)).to(dev)
+            eps_a = torch.zeros(Ca.shape[1], dtype=torch.float64, device=dev)
+        eps_b, Cb = eps_a.clone(), Ca.clone()
         diis = _DIIS(self.diis_max_vecs)
         E_old = 0.0
         na, nb = self.nalpha, self.nbeta
@@ -82,17 +105,19 @@ class UHF:
             Jt = Ja + Jb
             Fa = H + Jt - Ka
             Fb = H + Jt - Kb
-            E = enuc + 0.5 * (np.vdot(Da + Db, H) + np.vdot(Da, Fa) + np.vdot(Db, Fb))
-            ga = X.T @ (Fa @ Da @ S - S @ Da @ Fa) @ X
-            gb = X.T @ (Fb @ Db @ S - S @ Db @ Fb) @ X
-            err = np.concatenate([ga.ravel(), gb.ravel()])
-            d_rms = float(np.sqrt(np.mean(err ** 2)))
+            E = enuc + 0.5 * float(torch.sum((Da + Db) * H) + torch.sum(Da * Fa) + torch.sum(Db * Fb))
+            FDSa, FDSb = Fa @ Da @ S, Fb @ Db @ S
+            ga = X.T @ (FDSa - FDSa.T) @ X
+            gb = X.T @ (FDSb - FDSb.T) @ X
+            err = torch.cat([ga.reshape(-1), gb.reshape(-1)])
+            d_rms = float(torch.sqrt(torch.mean(err * err)))
             if abs(E - E_old) < self.e_conv and d_rms < self.d_conv:
-                self.Ca, self.Cb, self.Fa, self.Fb, self.eps_a, self.eps_b = Ca, Cb, Fa, Fb, eps_a, eps_b
-                return UHFResult(energy=E, Ca=Ca, Cb=Cb, Da=Da, Db=Db, iterations=it)
-            diis.add(np.stack([Fa, Fb]), err)
+                h = [x.cpu().numpy() for x in (Ca, Cb, Fa, Fb, eps_a, eps_b, Da, Db)]
+                self.Ca, self.Cb, self.Fa, self.Fb, self.eps_a, self.eps_b = h[:6]
+                return UHFResult(energy=E, Ca=h[0], Cb=h[1], Da=h[6], Db=h[7], iterations=it)
+            diis.add(torch.stack([Fa, Fb]), err)
             Fx = diis.extrapolate()
-            eps_a, Ca = eigh(Fx[0], S)
-            eps_b, Cb = eigh(Fx[1], S)
+            eps_a, Ca = diag(Fx[0])
+            eps_b, Cb = diag(Fx[1])
             E_old = E
         raise SCFConvergenceError(f"UHF did not converge in {self.maxiter} iterations")
