@@ -190,7 +190,9 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             const ACoord ac = a_coord<KS>(tid + NTHR * it, I0, J);
             const int off = sl.off[J * RT + ac.sub];
             const T* tp = off >= 0 ? BQ + off : ztile;
-            ra[it] = (I0 + ac.sub <= J) ? tp[ac.r * TILE + kh + ac.c] : tp[(kh + ac.c) * TILE + ac.r];
+            // B is streamed (non-temporal) so that the C panel every workgroup re-reads stays in L2
+            ra[it] = __builtin_nontemporal_load((I0 + ac.sub <= J) ? tp + ac.r * TILE + kh + ac.c
+                                                                   : tp + (kh + ac.c) * TILE + ac.r);
         }
         const T* Cj = C + static_cast<size_t>(J * TILE + kh) * ldc + col0;
 #pragma unroll
