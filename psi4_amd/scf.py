@@ -50,10 +50,6 @@ class SCFResult:
     timings: dict = field(default_factory=dict)
 
 
-def _vdot(a, b):
-    return float(torch.sum(a * b)) if torch.is_tensor(a) else float(np.vdot(a, b))
-
-
 def _copy(a):
     return a.clone() if torch.is_tensor(a) else a.copy()
 
@@ -65,11 +61,24 @@ class _DIIS:
         self.max_vecs = max_vecs
         self.F, self.err = [], []
 
+    def _gram(self):
+        """<err_i, err_j> for all stored vectors (device reductions, one sync)."""
+        n = len(self.err)
+        pairs = [(i, j) for i in range(n) for j in range(i, n)]
+        if torch.is_tensor(self.err[0]):
+            vals = torch.stack([torch.sum(self.err[i] * self.err[j]) for i, j in pairs]).cpu().numpy()
+        else:
+            vals = [float(np.vdot(self.err[i], self.err[j])) for i, j in pairs]
+        G = np.empty((n, n))
+        for (i, j), v in zip(pairs, vals):
+            G[i, j] = G[j, i] = v
+        return G
+
     def add(self, F, err):
         self.F.append(_copy(F))
         self.err.append(_copy(err))
         if len(self.F) > self.max_vecs:
-            worst = int(np.argmax([_vdot(e, e) for e in self.err]))
+            worst = int(np.argmax(np.diag(self._gram())))
             del self.F[worst]
             del self.err[worst]
 
@@ -77,9 +86,7 @@ class _DIIS:
         n = len(self.F)
         B = -np.ones((n + 1, n + 1))
         B[n, n] = 0.0
-        for i in range(n):
-            for j in range(i, n):
-                B[i, j] = B[j, i] = _vdot(self.err[i], self.err[j])
+        B[:n, :n] = self._gram()
         rhs = np.zeros(n + 1)
         rhs[n] = -1.0
         c = np.linalg.lstsq(B, rhs, rcond=None)[0]
