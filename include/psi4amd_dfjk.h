@@ -20,8 +20,10 @@
  *     handle's own stream is a blocking stream, so it is ordered with work on the
  *     legacy default (null) stream; a caller on a non-blocking stream passes that
  *     stream.  *_device calls return with their work queued (host-pointer calls
- *     return after it finished), except the signs test of a non-symmetric pair and
- *     the adjoint, which synchronise the stream.
+ *     return after it finished).  Two cases wait for the stream: the structure test
+ *     of a non-symmetric pair when the pair form is DFJK_FORM_DETECT (the launch
+ *     shapes depend on its outcome; declare the form with dfjk_set_pair_form to
+ *     skip it), and the adjoint, which returns E on the host.
  */
 #ifndef PSI4AMD_DFJK_H
 #define PSI4AMD_DFJK_H
@@ -53,8 +55,19 @@ enum dfjk_precision {
 };
 
 enum dfjk_fitting {
-    DFJK_FIT_CHOLESKY = 0, /* B = L^-1 (P|mn), J = L L^T (host FP64 Cholesky) */
-    DFJK_FIT_EIGEN = 1     /* B = J^-1/2 (P|mn) with psi4's condition cut (libmints/matrix.cc:2370-2424) */
+    DFJK_FIT_CHOLESKY = 0, /* B = L^-1 (P|mn), J = L L^T (FP64 blocked Cholesky, default) */
+    DFJK_FIT_EIGEN = 1     /* B = J^-1/2 (P|mn), J^-1/2 = V f(a) V^T with psi4's condition cut: eigenvalues
+                            * |a| < condition * max|a| and non-finite powers give 0 (libmints/matrix.cc:2370-2424,
+                            * applied by lib3index/dfhelper.cc:1491-1517); FP64 device eigensolver */
+};
+
+/* Structure of a non-symmetric density pair (no psi4 counterpart: psi4 always runs the general
+ * two-transform path).  DETECT tests the columns on the device (one host round trip); a caller
+ * that knows the structure declares it and the test is skipped. */
+enum dfjk_pair_form {
+    DFJK_FORM_DETECT = 0,   /* default: signed (Cr = Cl diag(+-1)), cross or general, found on the device */
+    DFJK_FORM_GENERAL = 1,  /* no structure assumed: two half-transforms, every output tile */
+    DFJK_FORM_CROSS = 2     /* Cl = [A, V], Cr = [V, A] (ncol even): dD = A V^T + V A^T, K symmetric */
 };
 
 enum dfjk_subalgorithm {      /* psi4 SCF_SUBTYPE (DFHelper subalgo_, dfhelper.h:370-376; jk.h:1039 set_subalgo) */
@@ -99,8 +112,27 @@ int dfjk_set_fitting(dfjk_handle* h, int fitting);         /* enum dfjk_fitting 
 int dfjk_set_stream(dfjk_handle* h, void* hip_stream);     /* NULL: the handle's own stream */
 int dfjk_set_profiling(dfjk_handle* h, int on);            /* per-kernel HIP-event timing */
 
-/* JK::memory_estimate (MemDFJK.cc:65-69): doubles of device memory this handle needs. */
+/* JK::memory_estimate (MemDFJK.cc:65-69) -> DFHelper::get_core_size (dfhelper.h:95-98, dfhelper.cc:216-236):
+ * naux * nsig + naux^2 + omp_nthread * nbf^2 + 3 * nbf^2 * Qshell_max doubles, where nsig is the number
+ * of Schwarz-significant function pairs and Qshell_max the largest auxiliary shell; this is the figure
+ * psi4's JK::build_JK compares with its memory to choose MEM_DF or DISK_DF (jk.cc:206-230).  As in
+ * MemDFJK, wK is not counted (the helper's do_wK is set later, in preiterations).  Needs no device. */
 int dfjk_memory_estimate(dfjk_handle* h, size_t* doubles);
+/* Doubles of device memory this handle keeps resident: B (and Bw with wK), the metric during
+ * the fit, D/J/K staging and the packed C panels (no psi4 counterpart). */
+int dfjk_device_footprint(dfjk_handle* h, size_t* doubles);
+/* The Schwarz function-pair mask of DFHelper::prepare_sparsity (dfhelper.cc:299-420):
+ * mask[m*nbf + n] = 1 when (mn|mn) >= cutoff^2 / max (mn|mn), else 0 (nbf x nbf bytes), and the
+ * cutoff it was built with.  Needs no device; either output may be NULL. */
+int dfjk_get_pair_mask(dfjk_handle* h, uint8_t* mask, double* cutoff);
+/* The metric fit in use: enum dfjk_fitting, the condition, and the number of metric
+ * eigenvalues kept (psi4's Matrix::power return value; naux for the Cholesky fit, 0 before
+ * initialize).  Any output may be NULL. */
+int dfjk_get_fit_info(dfjk_handle* h, int* fitting, double* condition, int* naux_kept);
+/* Declared structure of non-symmetric pairs (enum dfjk_pair_form); applies to later computes. */
+int dfjk_set_pair_form(dfjk_handle* h, int form);
+/* The auxiliary range set by dfjk_set_aux_range (all of [0, naux) by default). */
+int dfjk_get_aux_range(dfjk_handle* h, int* q0, int* q1);
 /* JK::initialize -> MemDFJK::preiterations -> DFHelper::initialize (dfhelper.cc:149-215):
  * Schwarz screening, metric, host FP64 factorisation, (P|mn), GPU fit into HBM. */
 int dfjk_initialize(dfjk_handle* h);

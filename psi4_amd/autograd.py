@@ -11,17 +11,32 @@ dE/dB is the three-index gradient density that psi4's DF-SCF gradient
 contracts with derivative integrals (scfgrad/jk_grad.cc:240-262, 868+ form the
 same contraction from (A|mn) D_mn and (A|ij) C_pi C_qi).
 
-B, when given, is the dense fitted tensor [naux][nbf][nbf] on the device; the
-forward uploads it into the JK object (replacing its resident tensor), so the
-energy is a function of that B.  It must be symmetric in its last two indices:
-the JK object stores one triangle, and dE/dB is the gradient with respect to
-each dense element at a symmetric point.  All arithmetic is FP64.
+B, when given, is the dense fitted tensor [naux][nbf][nbf] on the device.  The
+energy is then a function of that B alone: it is loaded into a tensor-level JK
+object of the same shape (no screening mask, so every element counts) that is
+kept beside the caller's JK and reused, and the caller's JK -- its resident
+tensor, auxiliary range and precision -- is left as it was.  B must be
+symmetric in its last two indices: the JK object stores one triangle, and dE/dB
+is the gradient with respect to each dense element at a symmetric point.  All
+arithmetic is FP64.
 """
 import torch
 
 
 def _sync():
     torch.cuda.current_stream().synchronize()
+
+
+def _tensor_twin(jk, nbf, naux):
+    """Tensor-level JK of jk's shape that holds a caller-supplied B (made once, kept on jk)."""
+    twin = getattr(jk, "_autograd_twin", None)
+    if twin is None:
+        from .core import JK
+        twin = JK.build_tensor(nbf, naux)
+        twin.set_print(0)
+        twin.initialize()
+        jk._autograd_twin = twin
+    return twin
 
 
 class _SCFEnergy2e(torch.autograd.Function):
@@ -38,26 +53,33 @@ class _SCFEnergy2e(torch.autograd.Function):
             if tuple(B.shape) != (info["naux"], n, n) or B.dtype != torch.float64 or not B.is_cuda:
                 raise ValueError("B must be a CUDA float64 naux x nbf x nbf tensor")
             B = B.contiguous()
+            jk = _tensor_twin(jk, n, info["naux"])
             _sync()
             jk.set_B_dense_device(B.data_ptr())
         _sync()
         E = jk.energy_adjoint_device(C.data_ptr(), C.shape[1], 0, 0)
         ctx.jk = jk
-        ctx.save_for_backward(C)
         ctx.with_B = B is not None
+        if B is not None:
+            ctx.save_for_backward(C, B)
+        else:
+            ctx.save_for_backward(C)
         return torch.tensor(E, dtype=torch.float64, device=C.device)
 
     @staticmethod
     def backward(ctx, gE):
-        (C,) = ctx.saved_tensors
+        C = ctx.saved_tensors[0]
         jk = ctx.jk
+        if ctx.with_B:
+            # the twin may hold the B of a later forward: load this one again
+            _sync()
+            jk.set_B_dense_device(ctx.saved_tensors[1].data_ptr())
         want_C = ctx.needs_input_grad[0]
         want_B = ctx.with_B and ctx.needs_input_grad[1]
         dC = torch.empty_like(C) if want_C else None
         dB = None
         if want_B:
-            info = jk.info()
-            jk.set_aux_range(0, info["naux"])
+            info = jk.info()                 # the twin: its whole auxiliary range is in use
             dB = torch.empty((info["naux"], info["nbf"], info["nbf"]), dtype=torch.float64, device=C.device)
         _sync()
         jk.energy_adjoint_device(C.data_ptr(), C.shape[1], dC.data_ptr() if want_C else 0,

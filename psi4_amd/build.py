@@ -1,7 +1,8 @@
 """In-tree build of the native libraries (gfx950).
 
   libp4a_ints.so       host integral engine + host FP64 linear algebra (g++, OpenMP)
-  libpsi4amd_dfjk.so   HIP kernels + host orchestration + the C ABI (hipcc, gfx950)
+  libpsi4amd_dfjk.so   HIP kernels + host orchestration + the C ABI (hipcc, gfx950; links
+                       rocSOLVER/rocBLAS for the eigen fit of the metric)
   _core*.so            pybind11 module over the C ABI (g++)
 
 Run ``python -m psi4_amd.build`` or ``__graft_entry__.build()``.
@@ -75,8 +76,9 @@ def build_dfjk(verbose=False):
             sys.stderr.write(out_txt)
             raise RuntimeError(f"build step failed: {cmd[0]} ... {cmd[-3]}")
     out = os.path.join(LIB, "libpsi4amd_dfjk.so")
-    _run([hipcc, *HIPCC_FLAGS, "-shared", *objs, f"-L{LIB}", "-lp4a_ints", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"],
-         verbose)
+    # rocSOLVER (dsyevd) for psi4's eigen fit of the metric; rocBLAS carries its handle
+    _run([hipcc, *HIPCC_FLAGS, "-shared", *objs, f"-L{LIB}", "-lp4a_ints", f"-L{ROCM}/lib", "-lrocsolver",
+          "-lrocblas", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib", "-o", out + ".tmp"], verbose)
     os.replace(out + ".tmp", out)
     return out
 

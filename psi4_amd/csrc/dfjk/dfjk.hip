@@ -18,6 +18,8 @@
 #include <string>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "../ints/basis_build.hpp"
 #include "../ints/ints.hpp"
 #include "../ints/hostlinalg.hpp"
@@ -150,9 +152,13 @@ struct dfjk_handle {
     bool do_J = true, do_K = true;
     bool do_wK = false;
     double omega = 0.0;
-    int nthreads = 1;
+    int nthreads = 1;      // host integral threads
+    int omp_nthread = 1;   // psi4's omp_nthread_ (jk.cc:254-257, default one thread): memory estimate
     int precision = DFJK_PREC_FP64;
     int fitting = DFJK_FIT_CHOLESKY;
+    int pair_form = DFJK_FORM_DETECT;
+    bool fit_lower = true;  // the metric factor is lower triangular (Cholesky): the fit skips its upper part
+    int fit_kept = 0;       // metric eigenvalues kept by the fit (naux for Cholesky)
     bool initialized = false;
     int aux_q0 = 0, aux_q1 = 0;
     // stream
@@ -200,6 +206,14 @@ struct dfjk_handle {
     // workspace
     DevBuf ws_craw, ws_cl, ws_cr, ws_D, ws_W, ws_dpart, ws_d, ws_Jpart, ws_XL, ws_XR, ws_P, ws_Pd, ws_J, ws_K, ws_G, ws_sign, ws_c32, ws_cbf, ws_cmap;
     DevBuf ws_ct, ws_red, ws_Y, ws_PC, ws_gam, ws_dc;
+    // column map of the half-transform: the device copy is re-uploaded (from pinned memory,
+    // asynchronously) only when it changes
+    std::vector<int> cmap_dev;          // what ws_cmap holds
+    int* cmap_pin = nullptr;            // pinned staging of the last upload
+    size_t cmap_pin_cap = 0;
+    hipEvent_t ev_cmap = nullptr;       // that upload has been read
+    int8_t* sign_pin = nullptr;         // pinned copy of the column-structure test
+    size_t sign_pin_cap = 0;
     // timing
     EventTimer timer;
     double t_ms[4] = {0, 0, 0, 0};
@@ -346,6 +360,27 @@ void prepare_sparsity(dfjk_handle* h) {
     for (size_t i = 0; i < shell_max.size(); ++i) h->shell_mask[i] = shell_max[i] >= tol;
 }
 
+// Screening and tile geometry on the host, once (no device needed).  A tensor-level handle has
+// no screening: every pair is significant.
+void build_tiles(dfjk_handle* h);
+void ensure_sparsity(dfjk_handle* h) {
+    if (!h->tiles.empty()) return;
+    if (h->tensor_mode) {
+        h->fun_mask.assign(static_cast<size_t>(h->nbf) * h->nbf, 1);
+        h->nsig = static_cast<int64_t>(h->nbf) * h->nbf;
+    } else {
+        prepare_sparsity(h);
+    }
+    build_tiles(h);
+}
+
+// Functions in the largest auxiliary shell (BasisSet::max_function_per_shell, DFHelper's Qshell_max_)
+int qshell_max(const dfjk_handle* h) {
+    int m = 1;
+    for (const Shell& sh : h->aux.shells) m = std::max(m, sh.nfunc());
+    return m;
+}
+
 // Metric factor on the host (FP64): returns M with B = M (P|mn).
 // Wall-clock laps of the set-up phases.
 struct Lap {
@@ -360,7 +395,7 @@ struct Lap {
 
 [[noreturn]] void cholesky_failed(int pivot) {
     fail(DFJK_ERR_ARG, "metric Cholesky failed at pivot " + std::to_string(pivot) +
-                           " (metric not positive definite; use DFJK_FIT_EIGEN)");
+                           " (metric not positive definite: set DFJK_FIT_EIGEN, psi4's truncating J^-1/2)");
 }
 
 // M = L^-1 with J = L L^T, on the device.  Right-looking blocked Cholesky: each NB x NB
@@ -435,7 +470,76 @@ std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<doubl
     return M;
 }
 
-// The metric factor M = L^-1; on the host only when want_host (else it is left in ws_G).
+// psi4's fit, J^-1/2 = Matrix::power(-0.5, condition) (libmints/matrix.cc:2370-2424, called from
+// dfhelper.cc:1491-1517): J = V diag(a) V^T by an FP64 symmetric eigensolver on the device
+// (rocSOLVER dsyevd, the counterpart of psi4's LAPACK DSYEV), f_i = a_i^-1/2 except that
+// |a_i| < condition * max(|a_min|, |a_max|) and non-finite powers give 0, then
+// M = V diag(f) V^T, symmetric and full (the fit runs over the whole k range).  M is left in
+// ws_G (dM_ready) and returned on the host only when want_host.
+std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double>& Jm, bool want_host) {
+    const int n = h->naux;
+    const size_t nn = static_cast<size_t>(n) * n;
+    const hipStream_t s = h->stream;
+    h->dM_ready = false;
+    DevBuf dA, dS, dRT, dw, de, df, dinfo;
+    double* A = dA.get<double>(nn);
+    double* w = dw.get<double>(n);
+    double* e = de.get<double>(n);
+    rocblas_int* info = dinfo.get<rocblas_int>(1);
+    hcheck(hipMemcpyAsync(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload metric");
+    rocblas_handle rb = nullptr;
+    if (rocblas_create_handle(&rb) != rocblas_status_success) fail(DFJK_ERR_DEVICE, "rocblas_create_handle failed");
+    struct RbGuard {
+        rocblas_handle r;
+        ~RbGuard() { rocblas_destroy_handle(r); }
+    } guard{rb};
+    if (rocblas_set_stream(rb, s) != rocblas_status_success) fail(DFJK_ERR_DEVICE, "rocblas_set_stream failed");
+    // the metric is symmetric, so its row-major storage is also its column-major storage
+    const rocblas_status st = rocsolver_dsyevd(rb, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, e, info);
+    if (st != rocblas_status_success)
+        fail(DFJK_ERR_DEVICE, std::string("rocsolver_dsyevd: ") + rocblas_status_to_string(st));
+    std::vector<double> a(n);
+    rocblas_int hinfo = 0;
+    hcheck(hipMemcpyAsync(a.data(), w, n * sizeof(double), hipMemcpyDeviceToHost, s), "eigenvalues");
+    hcheck(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s), "eigensolver status");
+    hcheck(hipStreamSynchronize(s), "metric eigensolver");
+    if (hinfo != 0) fail(DFJK_ERR_INTERNAL, "metric eigensolver did not converge (info " + std::to_string(hinfo) + ")");
+    // eigenvalues ascend: the largest magnitude is at one end
+    const double max_a = std::max(std::fabs(a[n - 1]), std::fabs(a[0]));
+    std::vector<double> f(n);
+    int kept = 0;
+    for (int i = 0; i < n; ++i) {
+        if (std::fabs(a[i]) < h->condition * max_a) {
+            f[i] = 0.0;
+        } else {
+            const double v = std::pow(a[i], -0.5);
+            f[i] = std::isfinite(v) ? v : 0.0;
+            kept += std::isfinite(v) ? 1 : 0;
+        }
+    }
+    h->fit_kept = kept;
+    double* fd = df.get<double>(n);
+    hcheck(hipMemcpyAsync(fd, f.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "upload powers");
+    // column-major eigenvectors read row-major: row i of A is eigenvector i (R).  M = R^T diag(f) R.
+    double* S = dS.get<double>(nn);
+    double* RT = dRT.get<double>(nn);
+    launch_scale_rows(A, n, n, fd, S, s);
+    launch_transpose(A, n, n, n, RT, n, s);
+    double* X = h->ws_G.get<double>(nn);
+    launch_gemm_nn_f64(RT, n, S, n, X, n, n, n, n, 1.0, 0.0, s);
+    h->dM_ready = true;
+    std::vector<double> M;
+    if (want_host) {
+        M.resize(nn);
+        hcheck(hipMemcpyAsync(M.data(), X, nn * sizeof(double), hipMemcpyDeviceToHost, s), "download M");
+    }
+    // the temporaries are freed on return: the products must have read them
+    hcheck(hipStreamSynchronize(s), "metric power");
+    return M;
+}
+
+// The metric factor M (L^-1, or psi4's truncated J^-1/2); on the host only when want_host
+// (else it is left in ws_G).
 std::vector<double> metric_factor(dfjk_handle* h, bool want_host) {
     const int na = h->naux;
     Lap lap;
@@ -443,6 +547,8 @@ std::vector<double> metric_factor(dfjk_handle* h, bool want_host) {
     metric(h->aux, Jm.data(), h->nthreads);
     h->setup_ms[1] = lap();
     std::vector<double> M;
+    h->fit_lower = h->fitting == DFJK_FIT_CHOLESKY;
+    h->fit_kept = na;
     if (h->fitting == DFJK_FIT_CHOLESKY) {
         if (std::getenv("PSI4AMD_HOST_FACTOR")) {
             h->dM_ready = false;
@@ -454,7 +560,7 @@ std::vector<double> metric_factor(dfjk_handle* h, bool want_host) {
             M = device_metric_factor(h, Jm, want_host);
         }
     } else {
-        fail(DFJK_ERR_ARG, "DFJK_FIT_EIGEN is not available in this build");
+        M = device_metric_eigen(h, Jm, want_host);
     }
     h->setup_ms[2] = lap();
     return M;
@@ -481,13 +587,13 @@ void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vect
                "upload (P|mn)");
         if (host_dst) {
             launch_gemm_nn_f64(dM, na, dA, static_cast<int>(w), dBp, static_cast<int>(w), na, static_cast<int>(w), na,
-                               1.0, 0.0, h->stream, 1, 0, 0, 0, 1);
+                               1.0, 0.0, h->stream, 1, 0, 0, 0, h->fit_lower ? 1 : 0);
             hcheck(hipMemcpy2DAsync(host_dst + c0, ncolB * sizeof(double), dBp, w * sizeof(double),
                                     w * sizeof(double), na, hipMemcpyDeviceToHost, h->stream),
                    "download fitted B");
         } else {
             launch_gemm_nn_f64(dM, na, dA, static_cast<int>(w), dBp + c0, static_cast<int>(ncolB), na,
-                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, 1);
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, h->fit_lower ? 1 : 0);
         }
         hcheck(hipStreamSynchronize(h->stream), "fit");
     }
@@ -518,7 +624,10 @@ bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
     const size_t abytes = static_cast<size_t>(h->naux) * ncolB * sizeof(double);
     size_t free_b = 0, total_b = 0;
     hcheck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    if (abytes + (h->dB.p ? 0 : abytes) + (size_t(2) << 30) > free_b) return false;
+    // the fit writes B (omega = 0) or the attenuated tensor: count it unless already allocated
+    const DevBuf& fit_dst = omega > 0.0 ? h->dBw : h->dB;
+    const size_t dst_bytes = fit_dst.bytes >= abytes ? 0 : abytes;
+    if (abytes + dst_bytes + (size_t(2) << 30) > free_b) return false;
 
     auto shells = [](const Basis& b, std::vector<GpuShell>& sh, std::vector<double>& e, std::vector<double>& c) {
         for (size_t i = 0; i < b.shells.size(); ++i) {
@@ -646,7 +755,7 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
         for (size_t c0 = 0; c0 < ncolB; c0 += cw) {
             const size_t w = std::min(cw, ncolB - c0);
             launch_gemm_nn_f64(dM, na, A + c0, static_cast<int>(ncolB), dBp + c0, static_cast<int>(ncolB), na,
-                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, 1);
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, h->fit_lower ? 1 : 0);
         }
     } else {
         size_t cw = std::max<size_t>(TILE2, (size_t(1) << 27) / std::max(1, na));
@@ -655,7 +764,7 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
         for (size_t c0 = 0; c0 < ncolB; c0 += cw) {
             const size_t w = std::min(cw, ncolB - c0);
             launch_gemm_nn_f64(dM, na, A + c0, static_cast<int>(ncolB), tmp, static_cast<int>(w), na,
-                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, 1);
+                               static_cast<int>(w), na, 1.0, 0.0, h->stream, 1, 0, 0, 0, h->fit_lower ? 1 : 0);
             hcheck(hipMemcpy2DAsync(host_dst + c0, ncolB * sizeof(double), tmp, w * sizeof(double),
                                     w * sizeof(double), na, hipMemcpyDeviceToHost, h->stream),
                    "download fitted B");
@@ -740,7 +849,15 @@ void combine_Bw(dfjk_handle* h) {
     }
 }
 
+// B resident on the device (in core): every path that launches on dB checks this first
+void require_device_B(const dfjk_handle* h, const char* what) {
+    if (h->ooc || !h->dB.p)
+        fail(DFJK_ERR_STATE, std::string(what) + " needs B resident on the device (subalgorithm INCORE); this handle "
+                                                 "streams B from host memory");
+}
+
 void build_Bw(dfjk_handle* h) {
+    require_device_B(h, "wK");  // dfhelper.cc:201-205: psi4 is not equipped to do wK out of core
     if (h->tensor_mode) fail(DFJK_ERR_STATE, "wK needs the attenuated tensor: dfjk_set_Bw_dense");
     if (!(h->omega > 0.0)) fail(DFJK_ERR_ARG, "wK needs omega > 0 (dfjk_set_omega)");
     if (h->fit_M.empty()) h->fit_M = metric_factor(h, true);
@@ -762,10 +879,7 @@ void build_Bw(dfjk_handle* h) {
 void initialize_basis(dfjk_handle* h) {
     for (double& t : h->setup_ms) t = 0.0;
     Lap lap;
-    if (h->tiles.empty()) {
-        prepare_sparsity(h);
-        build_tiles(h);
-    }
+    ensure_sparsity(h);
     upload_tiles(h);
     h->setup_ms[0] = lap();
     std::vector<double> M = metric_factor(h, h->do_wK);  // the host copy rebuilds Bw after an omega change
@@ -848,6 +962,60 @@ void invalidate_copies(dfjk_handle* h) {
     h->dBlT.release();
 }
 
+// Device workspace of one compute.  Allocating mode: the handle's growable buffers.  Fixed
+// mode (dfjk_compute_device_noalloc): a bump arena over this device's static scratch array;
+// nothing is allocated, and a request past the arena's end is an error.
+struct Work {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    bool fixed() const { return base != nullptr; }
+    template <class T>
+    T* get(DevBuf& b, size_t n) {
+        if (!base) return b.get<T>(n);
+        const size_t bytes = (std::max<size_t>(n * sizeof(T), 256) + 255) / 256 * 256;
+        if (bytes > cap - used)
+            fail(DFJK_ERR_MEMORY, "dfjk_compute_device_noalloc: this system needs more than the " +
+                                      std::to_string(cap >> 20) + " MiB scratch array");
+        T* p = reinterpret_cast<T*>(base + used);
+        used += bytes;
+        return p;
+    }
+    size_t left() const { return cap - used; }
+};
+
+// Column map of the half-transform on the device.  Allocating mode: uploaded only when it
+// differs from what the device buffer holds.  Either way the copy is made from pinned memory on
+// the handle's stream (no host wait; the pinned staging is rewritten only after the event of
+// its previous upload has passed).
+int* upload_cmap(dfjk_handle* h, const std::vector<int>& cmap, Work& wk) {
+    const size_t n = cmap.size();
+    int* dmap;
+    if (wk.fixed()) {
+        dmap = wk.get<int>(h->ws_cmap, n);  // the scratch array is shared: always upload
+        if (h->cmap_pin_cap < n) fail(DFJK_ERR_MEMORY, "dfjk_compute_device_noalloc: column map too long");
+    } else {
+        void* before = h->ws_cmap.p;
+        dmap = h->ws_cmap.get<int>(n);
+        if (h->ws_cmap.p != before) h->cmap_dev.clear();  // reallocated: contents gone
+        if (h->cmap_dev == cmap) return dmap;
+    }
+    if (!h->ev_cmap) hcheck(hipEventCreateWithFlags(&h->ev_cmap, hipEventDisableTiming), "event");
+    hcheck(hipEventSynchronize(h->ev_cmap), "column map staging");
+    if (h->cmap_pin_cap < n) {
+        if (h->cmap_pin) hcheck(hipHostFree(h->cmap_pin), "hipHostFree");
+        h->cmap_pin = nullptr;
+        h->cmap_pin_cap = 0;
+        hcheck(hipHostMalloc(reinterpret_cast<void**>(&h->cmap_pin), std::max<size_t>(n, 1024) * sizeof(int), 0),
+               "hipHostMalloc");
+        h->cmap_pin_cap = std::max<size_t>(n, 1024);
+    }
+    std::memcpy(h->cmap_pin, cmap.data(), n * sizeof(int));
+    hcheck(hipMemcpyAsync(dmap, h->cmap_pin, n * sizeof(int), hipMemcpyHostToDevice, h->stream), "column map");
+    hcheck(hipEventRecord(h->ev_cmap, h->stream), "record");
+    h->cmap_dev = wk.fixed() ? std::vector<int>() : cmap;
+    return dmap;
+}
+
 // One density pair: Cl/Cr are device [nbf][ncol] row-major (ld = ncol).
 // Cr == nullptr: lr_symmetric.  Two structured non-symmetric forms are recognised
 // (bitwise column tests on the device) and kept symmetric:
@@ -860,8 +1028,9 @@ void invalidate_copies(dfjk_handle* h) {
 //    product carries the small factor V).  K = sum_Q X_A X_V^T + X_V X_A^T: one
 //    half-transform of [A, V], two contractions of the upper block-triangle with the
 //    column halves swapped between the operands.
+// The cross form can also be declared (dfjk_set_pair_form), which skips the test.
 void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol, double* dD, double* dJ, double* dK,
-                  int accumulate) {
+                  int accumulate, Work& wk) {
     const TileGeom g = h->geom();
     const hipStream_t s = h->stream;
     const int q0 = h->aux_q0, q1 = h->aux_q1;
@@ -869,29 +1038,41 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     const int nbf = h->nbf, nbfp = h->nbfp;
     const int xrows = (h->nT + 3) / 4 * 128;
     EventTimer& tm = h->timer;
+    if (wk.fixed() && h->ooc) fail(DFJK_ERR_STATE, "dfjk_compute_device_noalloc needs B in core");
 
-    // classify the pair and build the column map
+    // classify the pair (or take the declared form) and build the column map
     bool sym = (dCr == nullptr), cross = false;
     std::vector<int> hs(ncol, 1);
-    if (!sym) {
-        int8_t* sg = h->ws_sign.get<int8_t>(ncol);
-        std::vector<int8_t> h8(ncol);
-        auto signs = [&](int shift) {
-            launch_column_signs(dCl, dCr, ncol, nbf, ncol, shift, sg, s);
-            hcheck(hipMemcpyAsync(h8.data(), sg, ncol, hipMemcpyDeviceToHost, s), "signs");
-            hcheck(hipStreamSynchronize(s), "signs");
-        };
-        signs(0);
+    if (!sym && h->pair_form == DFJK_FORM_CROSS) {
+        if (ncol % 2) fail(DFJK_ERR_ARG, "JK: the cross pair form needs an even column count");
+        cross = true;
+    } else if (!sym && h->pair_form == DFJK_FORM_DETECT) {
+        // both column tests in one round trip: shift 0 (signed form), shift ncol/2 (cross form)
+        const int ntest = ncol % 2 == 0 ? 2 : 1;
+        int8_t* sg = wk.get<int8_t>(h->ws_sign, 2 * static_cast<size_t>(ncol));
+        launch_column_signs(dCl, dCr, ncol, nbf, ncol, 0, sg, s);
+        if (ntest == 2) launch_column_signs(dCl, dCr, ncol, nbf, ncol, ncol / 2, sg + ncol, s);
+        if (h->sign_pin_cap < 2 * static_cast<size_t>(ncol)) {
+            if (wk.fixed()) fail(DFJK_ERR_MEMORY, "dfjk_compute_device_noalloc: too many columns for the structure test");
+            if (h->sign_pin) hcheck(hipHostFree(h->sign_pin), "hipHostFree");
+            h->sign_pin = nullptr;
+            h->sign_pin_cap = 0;
+            const size_t cap = std::max<size_t>(2 * static_cast<size_t>(ncol), 4096);
+            hcheck(hipHostMalloc(reinterpret_cast<void**>(&h->sign_pin), cap, 0), "hipHostMalloc");
+            h->sign_pin_cap = cap;
+        }
+        hcheck(hipMemcpyAsync(h->sign_pin, sg, static_cast<size_t>(ntest) * ncol, hipMemcpyDeviceToHost, s), "signs");
+        hcheck(hipStreamSynchronize(s), "signs");
+        const int8_t* h8 = h->sign_pin;
         bool allpm = true;
         for (int i = 0; i < ncol; ++i) {
             hs[i] = h8[i];
             allpm &= h8[i] != 0;
         }
         sym = allpm;
-        if (!sym && ncol % 2 == 0) {
-            signs(ncol / 2);
+        if (!sym && ntest == 2) {
             cross = true;
-            for (int i = 0; i < ncol; ++i) cross &= h8[i] == 1;
+            for (int i = 0; i < ncol; ++i) cross &= h8[ncol + i] == 1;
             if (cross) std::fill(hs.begin(), hs.end(), 1);
         }
     }
@@ -923,14 +1104,13 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     } else {
         for (int i = 0; i < ncol; ++i) cmap[i] = i;
     }
-    int* dmap = h->ws_cmap.get<int>(ldc);
-    hcheck(hipMemcpy(dmap, cmap.data(), ldc * sizeof(int), hipMemcpyHostToDevice), "column map");
+    int* dmap = upload_cmap(h, cmap, wk);
 
-    double* cl = h->ws_cl.get<double>(static_cast<size_t>(nbfp) * ldc);
+    double* cl = wk.get<double>(h->ws_cl, static_cast<size_t>(nbfp) * ldc);
     launch_pack_cmap_f64(dCl, ncol, nbf, dmap, cl, nbfp, ldc, s);
     double* cr = cl;
     if (dCr) {
-        cr = h->ws_cr.get<double>(static_cast<size_t>(nbfp) * ldc);
+        cr = wk.get<double>(h->ws_cr, static_cast<size_t>(nbfp) * ldc);
         launch_pack_cmap_f64(dCr, ncol, nbf, dmap, cr, nbfp, ldc, s);
     }
 
@@ -939,8 +1119,8 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     double* Dm = dD;
     if (need_D) {
         hipEvent_t e0 = tm.begin();
-        if (!Dm) Dm = h->ws_D.get<double>(static_cast<size_t>(nbf) * nbf);
-        double* P = h->ws_Pd.get<double>(static_cast<size_t>(h->ntile_all) * 16384);
+        if (!Dm) Dm = wk.get<double>(h->ws_D, static_cast<size_t>(nbf) * nbf);
+        double* P = wk.get<double>(h->ws_Pd, static_cast<size_t>(h->ntile_all) * 16384);
         launch_gemm_nt_f64(cl, 0, ldc, cr, 0, ldc, nbf, nbf, ldc, 1, 1, h->d_otiles_all.as<int2>(), h->ntile_all, P, 0,
                            s);
         launch_tile_reduce(P, 1, h->ntile_all, h->d_otile_of_all.as<int>(), h->nB, 0, nbf, nbf, Dm, nbf, 1.0, 0, s);
@@ -960,44 +1140,41 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     const size_t qs = static_cast<size_t>(h->ntp) * TILE2;  // elements per Q
     const size_t xq = static_cast<size_t>(xrows) * ldc;      // X elements per Q
     const size_t esz = prec == DFJK_PREC_FP64 ? 8 : 4;       // FP64, FP32, or BF16 hi+lo
-    // bytes of X workspace (16 GiB FP64, 4 GiB in the mixed modes, whose contraction runs
-    // best on smaller Q-blocks), or what the memory setting leaves beyond the resident tensor
-    size_t budget = prec == DFJK_PREC_FP64 ? g_x_budget : g_x_budget_mixed;
-    budget = memory_limited(h, budget,
-                            (h->ooc ? 0 : b_doubles(h)) * sizeof(double) +
-                                static_cast<size_t>(h->naux) * h->naux * sizeof(double));
-    int qblk = nq_total;
-    if (doK) {
-        const size_t per_q = xq * esz * (general ? 2 : 1);
-        qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
-        qblk = std::min(qblk, 65535);
-    }
-    if (h->ooc) {
-        // two FP64 stage buffers (plus their converted copies) share the budget with X
-        const size_t per_q = qs * (sizeof(double) + (prec == DFJK_PREC_FP64 ? 0 : 2 * sizeof(float))) * 2;
-        const size_t stage_budget = std::max<size_t>(per_q, std::min<size_t>(size_t(8) << 30, budget / 2));
-        qblk = std::min<int>(qblk, static_cast<int>(std::max<size_t>(1, stage_budget / per_q)));
-    }
-    const int nblocks = (nq_total + qblk - 1) / qblk;
-    qblk = (nq_total + nblocks - 1) / nblocks;
 
-    // J workspace: the symmetrised tile-packed density and per-block partials
+    // resident lower-precision copies (in core); the fixed-workspace entry point allocates
+    // none, so an earlier allocating compute at this precision must have made them
+    if (!h->ooc) {
+        const bool miss = (prec == DFJK_PREC_FP32 && !h->dB32.p) ||
+                          (prec == DFJK_PREC_BF16X3 && (!h->dBh.p || (doK && !h->dBhT.p)));
+        if (wk.fixed() && miss)
+            fail(DFJK_ERR_STATE, "dfjk_compute_device_noalloc: the lower-precision copy of B for this precision does "
+                                 "not exist yet; run one dfjk_compute_device at this precision first");
+        if (prec == DFJK_PREC_FP32) ensure_B32(h);
+        if (prec == DFJK_PREC_BF16X3) {
+            ensure_Bbf(h);
+            if (doK) ensure_BbfT(h);
+        }
+    }
+
+    // J workspace in core: the symmetrised tile-packed density, d_Q, the d_Q partials per
+    // tile pair and the J partials per Q-chunk (J is one pass over the range, below)
     double* W = nullptr;
     double *dpart = nullptr, *dvec = nullptr, *Jpart = nullptr;
-    int jchunk = 0, jnchunk = 0;
+    int jchunk = 0, jnchunk = 0, qchunk = 0;
     if (doJ) {
-        hipEvent_t e0 = tm.begin();
-        W = h->ws_W.get<double>(qs);
-        launch_pack_w(g, Dm, nbf, W, s);
-        tm.end(T_J, e0);
-        dvec = h->ws_d.get<double>(h->naux);  // d_Q at its absolute Q (the adjoint reads it afterwards)
-        if (h->ooc) {
-            dpart = h->ws_dpart.get<double>(static_cast<size_t>(h->ntp) * qblk);
-            const int want_wg = 4096;
-            jnchunk = std::max(1, std::min(qblk, (want_wg + h->ntp - 1) / h->ntp));
-            jchunk = (qblk + jnchunk - 1) / jnchunk;
-            jnchunk = (qblk + jchunk - 1) / jchunk;
-            Jpart = h->ws_Jpart.get<double>(static_cast<size_t>(jnchunk) * qs);
+        W = wk.get<double>(h->ws_W, qs);
+        dvec = wk.get<double>(h->ws_d, h->naux);  // d_Q at its absolute Q (the adjoint reads it afterwards)
+        if (!h->ooc) {
+            dpart = wk.get<double>(h->ws_dpart, static_cast<size_t>(h->ntp) * h->naux);
+            // about 4096 workgroups for the second pass (fewer chunks in a fixed workspace)
+            int want_wg = 4096;
+            if (wk.fixed())
+                want_wg = static_cast<int>(std::max<size_t>(
+                    h->ntp, std::min<size_t>(4096, wk.left() / 8 / (TILE2 * sizeof(double)))));
+            jnchunk = std::max(1, std::min(nq_total, (want_wg + h->ntp - 1) / h->ntp));
+            qchunk = (h->naux + jnchunk - 1) / jnchunk;
+            jnchunk = (h->naux + qchunk - 1) / qchunk;
+            Jpart = wk.get<double>(h->ws_Jpart, static_cast<size_t>(jnchunk) * qs);
         }
     }
 
@@ -1005,8 +1182,41 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     const bool upper = sym || cross;  // symmetric K: contract the upper block-triangle only
     const int ntile = upper ? h->ntile_sym : h->ntile_all;
     const int2* otiles = upper ? h->d_otiles_sym.as<int2>() : h->d_otiles_all.as<int2>();
-    const int nsplit = doK ? choose_nsplit(ntile, qblk) : 1;
-    double* P = doK ? h->ws_P.get<double>(static_cast<size_t>(nsplit) * ntile * 16384) : nullptr;
+    const size_t per_q = xq * esz * (general ? 2 : 1);  // X bytes per Q
+    const size_t csz = static_cast<size_t>(nbfp) * ldc;
+    int qblk = nq_total, nsplit = 1;
+    double* P = nullptr;
+    if (!wk.fixed()) {
+        // bytes of X workspace (16 GiB FP64, 4 GiB in the mixed modes, whose contraction runs
+        // best on smaller Q-blocks), or what the memory setting leaves beyond the resident tensor
+        size_t budget = prec == DFJK_PREC_FP64 ? g_x_budget : g_x_budget_mixed;
+        budget = memory_limited(h, budget,
+                                (h->ooc ? 0 : b_doubles(h)) * sizeof(double) +
+                                    static_cast<size_t>(h->naux) * h->naux * sizeof(double));
+        if (doK) {
+            qblk = static_cast<int>(std::min<size_t>(nq_total, std::max<size_t>(1, budget / per_q)));
+            qblk = std::min(qblk, 65535);
+        }
+        if (h->ooc) {
+            // two FP64 stage buffers (plus their converted copies) share the budget with X
+            const size_t pq = qs * (sizeof(double) + (prec == DFJK_PREC_FP64 ? 0 : 2 * sizeof(float))) * 2;
+            const size_t stage_budget = std::max<size_t>(pq, std::min<size_t>(size_t(8) << 30, budget / 2));
+            qblk = std::min<int>(qblk, static_cast<int>(std::max<size_t>(1, stage_budget / pq)));
+        }
+        const int nblocks0 = (nq_total + qblk - 1) / qblk;
+        qblk = (nq_total + nblocks0 - 1) / nblocks0;
+        if (doK) {
+            nsplit = choose_nsplit(ntile, qblk);
+            P = wk.get<double>(h->ws_P, static_cast<size_t>(nsplit) * ntile * 16384);
+        }
+    } else if (doK) {
+        // fixed workspace: the K partials take at most a quarter of what is left, the packed C
+        // copies their share, and X the rest
+        const size_t tile_bytes = static_cast<size_t>(ntile) * 16384 * sizeof(double);
+        const int cap_split = static_cast<int>(std::max<size_t>(1, wk.left() / 4 / tile_bytes));
+        nsplit = std::min(choose_nsplit(ntile, std::min(nq_total, 65535)), cap_split);
+        P = wk.get<double>(h->ws_P, static_cast<size_t>(nsplit) * ntile * 16384);
+    }
     // contraction passes over k: (kbeg, kend, alpha, column offsets of the two operands)
     struct Pass {
         int kbeg, kend;
@@ -1031,9 +1241,8 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     uint16_t *cth_l = nullptr, *ctl_l = nullptr, *cth_r = nullptr, *ctl_r = nullptr;
     char *XL = nullptr, *XR = nullptr;
     if (doK) {
-        const size_t csz = static_cast<size_t>(nbfp) * ldc;
         if (prec == DFJK_PREC_FP32) {
-            float* a = h->ws_c32.get<float>(2 * csz);
+            float* a = wk.get<float>(h->ws_c32, 2 * csz);
             launch_pack_cmap_f32(dCl, ncol, nbf, dmap, a, nbfp, ldc, s);
             c32l = c32r = a;
             if (general) {
@@ -1041,7 +1250,7 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                 c32r = a + csz;
             }
         } else if (prec == DFJK_PREC_BF16X3) {
-            uint16_t* a = h->ws_cbf.get<uint16_t>(4 * csz);
+            uint16_t* a = wk.get<uint16_t>(h->ws_cbf, 4 * csz);
             cth_l = a;
             ctl_l = a + csz;
             launch_pack_cT_bf(dCl, ncol, nbf, dmap, cth_l, ctl_l, nbfp, ldc, s);
@@ -1053,18 +1262,34 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                 launch_pack_cT_bf(dCr, ncol, nbf, dmap, cth_r, ctl_r, nbfp, ldc, s);
             }
         }
-        XL = h->ws_XL.get<char>(xq * esz * qblk);
-        XR = general ? h->ws_XR.get<char>(xq * esz * qblk) : XL;
+        if (wk.fixed()) {
+            const size_t room = wk.left() > 1024 ? wk.left() - 1024 : 0;  // alignment of the two X arrays
+            qblk = static_cast<int>(std::min<size_t>(std::min(nq_total, 65535), room / per_q));
+            if (qblk < 1)
+                fail(DFJK_ERR_MEMORY, "dfjk_compute_device_noalloc: one auxiliary function's half-transform does not "
+                                      "fit the " + std::to_string(wk.cap >> 20) + " MiB scratch array");
+            const int nb0 = (nq_total + qblk - 1) / qblk;
+            qblk = (nq_total + nb0 - 1) / nb0;
+        }
+        XL = wk.get<char>(h->ws_XL, xq * esz * qblk);
+        XR = general ? wk.get<char>(h->ws_XR, xq * esz * qblk) : XL;
+    }
+    const int nblocks = (nq_total + qblk - 1) / qblk;
+    if (doJ && h->ooc) {
+        // out of core: d_Q partials and J partials per streamed block
+        dpart = wk.get<double>(h->ws_dpart, static_cast<size_t>(h->ntp) * qblk);
+        const int want_wg = 4096;
+        jnchunk = std::max(1, std::min(qblk, (want_wg + h->ntp - 1) / h->ntp));
+        jchunk = (qblk + jnchunk - 1) / jnchunk;
+        jnchunk = (qblk + jchunk - 1) / jchunk;
+        Jpart = wk.get<double>(h->ws_Jpart, static_cast<size_t>(jnchunk) * qs);
+    }
+    if (doJ) {
+        hipEvent_t e0 = tm.begin();
+        launch_pack_w(g, Dm, nbf, W, s);
+        tm.end(T_J, e0);
     }
 
-    // resident sources (in core)
-    if (!h->ooc) {
-        if (prec == DFJK_PREC_FP32) ensure_B32(h);
-        if (prec == DFJK_PREC_BF16X3) {
-            ensure_Bbf(h);
-            if (doK) ensure_BbfT(h);
-        }
-    }
     // In core, J is one pass over the whole range (two streaming kernels over the resident
     // tensor, or its lower-precision copy); out of core it is accumulated block by block below.
     if (doJ && !h->ooc) {
@@ -1080,24 +1305,17 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             js.a = h->dB.p;
         }
         hipEvent_t e0 = tm.begin();
-        double* dp = h->ws_dpart.get<double>(static_cast<size_t>(h->ntp) * h->naux);
-        if (q0 > 0 || q1 < h->naux)
-            hcheck(hipMemsetAsync(dp, 0, sizeof(double) * static_cast<size_t>(h->ntp) * h->naux, s), "memset");
-        launch_j1(g, js, W, h->naux, q0, q1, dp, s);
-        launch_j1_reduce(g, dp, h->naux, dvec, s);
-        const int want_wg = 4096;
-        int nchunk = std::max(1, std::min(nq_total, (want_wg + h->ntp - 1) / h->ntp));
-        const int qchunk = (h->naux + nchunk - 1) / nchunk;
-        nchunk = (h->naux + qchunk - 1) / qchunk;
-        double* Jp = h->ws_Jpart.get<double>(static_cast<size_t>(nchunk) * qs);
-        launch_j2(g, js, dvec, h->naux, qchunk, Jp, s);
-        launch_j_unpack(g, Jp, nchunk, dJ, nbf, 1.0, accumulate, s);
+        if (q0 > 0 || q1 < h->naux) launch_zero(dpart, static_cast<size_t>(h->ntp) * h->naux, s);
+        launch_j1(g, js, W, h->naux, q0, q1, dpart, s);
+        launch_j1_reduce(g, dpart, h->naux, dvec, s);
+        launch_j2(g, js, dvec, h->naux, qchunk, Jpart, s);
+        launch_j_unpack(g, Jpart, jnchunk, dJ, nbf, 1.0, accumulate, s);
         tm.end(T_J, e0);
     }
     auto stage_copy = [&](int blk) {  // out of core: host block -> stage buffer blk & 1
         const int qa = q0 + blk * qblk;
         const int nq = std::min(qblk, q1 - qa);
-        double* dst = h->ws_stage[blk & 1].get<double>(qs * qblk);
+        double* dst = wk.get<double>(h->ws_stage[blk & 1], qs * qblk);
         hcheck(hipStreamWaitEvent(h->cstream, h->ev_free[blk & 1], 0), "wait stage");
         hcheck(hipMemcpyAsync(dst, h->hB + static_cast<size_t>(qa) * qs, qs * nq * sizeof(double),
                               hipMemcpyHostToDevice, h->cstream),
@@ -1137,11 +1355,11 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             b64 = st;
             hipEvent_t e0 = tm.begin();
             if (prec == DFJK_PREC_FP32) {
-                float* c = h->ws_stage32.get<float>(qs * qblk);
+                float* c = wk.get<float>(h->ws_stage32, qs * qblk);
                 launch_to_f32(st, c, nel, s);
                 b32 = c;
             } else if (prec == DFJK_PREC_BF16X3) {
-                uint16_t* c = h->ws_stagebf.get<uint16_t>(4 * qs * qblk);
+                uint16_t* c = wk.get<uint16_t>(h->ws_stagebf, 4 * qs * qblk);
                 const size_t cap = qs * qblk;
                 launch_to_bf_split(st, c, c + cap, nel, s);
                 bh = c;
@@ -1225,9 +1443,8 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                 h->n_kgemm += 1;
             }
             if (nsp < nsplit && blk == 0)
-                hcheck(hipMemsetAsync(P + static_cast<size_t>(nsp) * ntile * 16384, 0,
-                                      sizeof(double) * static_cast<size_t>(nsplit - nsp) * ntile * 16384, s),
-                       "memset");
+                launch_zero(P + static_cast<size_t>(nsp) * ntile * 16384,
+                            static_cast<size_t>(nsplit - nsp) * ntile * 16384, s);
             tm.end(T_KGEMM, e1);
         }
         if (h->ooc) hcheck(hipEventRecord(h->ev_free[blk & 1], s), "record");
@@ -1256,8 +1473,8 @@ void compute_wk_pair(dfjk_handle* h, const double* dCl, const double* dCr, int n
     ht_panel_shape(ncol, &nbw, &npanel, &ldc);
     std::vector<int> cmap(ldc, -1);
     for (int i = 0; i < ncol; ++i) cmap[i] = i;
-    int* dmap = h->ws_cmap.get<int>(ldc);
-    hcheck(hipMemcpy(dmap, cmap.data(), ldc * sizeof(int), hipMemcpyHostToDevice), "column map");
+    Work alloc_wk;
+    int* dmap = upload_cmap(h, cmap, alloc_wk);
     double* cl = h->ws_cl.get<double>(static_cast<size_t>(nbfp) * ldc);
     launch_pack_cmap_f64(dCl, ncol, nbf, dmap, cl, nbfp, ldc, s);
     double* cr = cl;
@@ -1323,10 +1540,20 @@ void reset_timers(dfjk_handle* h) {
     ensure_stream(h);
 }
 
+// scratch: null for the allocating entry points, else this device's static scratch array
+// (dfjk_compute_device_noalloc; device pointers only, no timing events)
 void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
-                  double* const* D, double* const* J, double* const* K, int accumulate, bool device) {
+                  double* const* D, double* const* J, double* const* K, int accumulate, bool device,
+                  char* scratch = nullptr, size_t scratch_bytes = 0) {
     require_state(h, true);
     if (n < 0 || (n > 0 && (!Cl || !ncol))) fail(DFJK_ERR_ARG, "JK: bad C_left arguments");
+    const bool timing = h->timer.on;
+    if (scratch) h->timer.on = false;
+    struct TimerRestore {
+        dfjk_handle* h;
+        bool on;
+        ~TimerRestore() { h->timer.on = on; }
+    } restore{h, timing};
     reset_timers(h);
     const size_t nn = static_cast<size_t>(h->nbf) * h->nbf;
     for (int i = 0; i < n; ++i) {
@@ -1341,7 +1568,7 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
             // zero() left it; the exchange comes back in wK
             if (!accumulate) {
                 if (device)
-                    hcheck(hipMemsetAsync(dK, 0, nn * sizeof(double), h->stream), "memset");
+                    launch_zero(dK, nn, h->stream);
                 else
                     std::memset(dK, 0, nn * sizeof(double));
             }
@@ -1350,9 +1577,9 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
         if (ncol[i] == 0) {
             // psi4: D = 0, J += 0, K skipped for nocc == 0 (dfhelper.cc:3355-3358)
             if (device) {
-                if (dD) hcheck(hipMemsetAsync(dD, 0, nn * sizeof(double), h->stream), "memset");
-                if (dJ && !accumulate) hcheck(hipMemsetAsync(dJ, 0, nn * sizeof(double), h->stream), "memset");
-                if (dK && !accumulate) hcheck(hipMemsetAsync(dK, 0, nn * sizeof(double), h->stream), "memset");
+                if (dD) launch_zero(dD, nn, h->stream);
+                if (dJ && !accumulate) launch_zero(dJ, nn, h->stream);
+                if (dK && !accumulate) launch_zero(dK, nn, h->stream);
             } else {
                 if (dD) std::memset(dD, 0, nn * sizeof(double));
                 if (dJ && !accumulate) std::memset(dJ, 0, nn * sizeof(double));
@@ -1381,7 +1608,10 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
                 if (gK) hcheck(hipMemcpyAsync(gK, dK, nn * sizeof(double), hipMemcpyHostToDevice, h->stream), "up");
             }
         }
-        compute_pair(h, cl, cr, ncol[i], gD, gJ, gK, accumulate);
+        Work wk;  // per pair: the pairs run in stream order, so a fixed arena is reused
+        wk.base = scratch;
+        wk.cap = scratch_bytes;
+        compute_pair(h, cl, cr, ncol[i], gD, gJ, gK, accumulate, wk);
         if (!device) {
             if (dD) hcheck(hipMemcpyAsync(dD, gD, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download D");
             if (dJ && h->do_J)
@@ -1399,6 +1629,7 @@ void compute_wk_impl(dfjk_handle* h, int n, const double* const* Cl, const doubl
                      double* const* wK, int accumulate, bool device) {
     require_state(h, true);
     if (n < 0 || (n > 0 && (!Cl || !ncol || !wK))) fail(DFJK_ERR_ARG, "JK: bad wK arguments");
+    require_device_B(h, "wK");
     if (h->tensor_mode) {
         if (!h->dBw.p) fail(DFJK_ERR_STATE, "wK needs the attenuated tensor: dfjk_set_Bw_dense");
         if (h->bw_comb != h->wcombine || (h->wcombine && (h->bw_alpha != h->omega_alpha || h->bw_beta != h->omega_beta)))
@@ -1474,7 +1705,8 @@ void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double*
     h->precision = DFJK_PREC_FP64;
     h->do_J = h->do_K = true;
     try {
-        compute_pair(h, dC, nullptr, nocc, D, Jm, Km, 0);
+        Work wk;
+        compute_pair(h, dC, nullptr, nocc, D, Jm, Km, 0, wk);
     } catch (...) {
         h->aux_q0 = uq0, h->aux_q1 = uq1, h->precision = prec, h->do_J = dj, h->do_K = dk;
         throw;
@@ -1508,8 +1740,8 @@ void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double*
     ht_panel_shape(nocc, &nbw, &npanel, &ldc);
     std::vector<int> cmap(ldc, -1);
     for (int i = 0; i < nocc; ++i) cmap[i] = i;
-    int* dmap = h->ws_cmap.get<int>(ldc);
-    hcheck(hipMemcpy(dmap, cmap.data(), ldc * sizeof(int), hipMemcpyHostToDevice), "column map");
+    Work alloc_wk;
+    int* dmap = upload_cmap(h, cmap, alloc_wk);
     double* cl = h->ws_cl.get<double>(static_cast<size_t>(nbfp) * ldc);
     launch_pack_cmap_f64(dC, nocc, nbf, dmap, cl, nbfp, ldc, s);
     double* CT = h->ws_ct.get<double>(static_cast<size_t>(nocc) * nbf);
@@ -1607,6 +1839,9 @@ int dfjk_destroy(dfjk_handle* h) {
             if (h->ev_free[i]) (void)hipEventDestroy(h->ev_free[i]);
         }
         if (h->hB) (void)hipHostFree(h->hB);
+        if (h->cmap_pin) (void)hipHostFree(h->cmap_pin);
+        if (h->sign_pin) (void)hipHostFree(h->sign_pin);
+        if (h->ev_cmap) (void)hipEventDestroy(h->ev_cmap);
         delete h;
         if (cs) (void)hipStreamDestroy(cs);
         if (s) (void)hipStreamDestroy(s);
@@ -1623,6 +1858,11 @@ int dfjk_set_cutoff(dfjk_handle* h, double cutoff) {
     return guarded([&] {
         require_state(h, false);
         if (!(cutoff >= 0.0)) fail(DFJK_ERR_ARG, "cutoff must be >= 0");
+        if (!h->initialized && cutoff != h->cutoff) {
+            // a mask built for an earlier estimate is rebuilt with the new cutoff
+            h->tiles.clear();
+            h->fun_mask.clear();
+        }
         h->cutoff = cutoff;
     });
 }
@@ -1647,6 +1887,8 @@ int dfjk_set_do_K(dfjk_handle* h, int v) {
 int dfjk_set_do_wK(dfjk_handle* h, int v) {
     return guarded([&] {
         require_state(h, false);
+        if (v && h->initialized && h->ooc)  // dfhelper.cc:201-205
+            fail(DFJK_ERR_STATE, "DFJK: not equipped to do wK out of core; supply more memory or disable wK");
         h->do_wK = v != 0;
     });
 }
@@ -1697,6 +1939,7 @@ int dfjk_set_omp_nthread(dfjk_handle* h, int n) {
         require_state(h, false);
         if (n < 1) fail(DFJK_ERR_ARG, "omp_nthread must be >= 1");
         h->nthreads = n;
+        h->omp_nthread = n;
     });
 }
 int dfjk_set_precision(dfjk_handle* h, int p) {
@@ -1740,24 +1983,64 @@ int dfjk_memory_estimate(dfjk_handle* h, size_t* doubles) {
     return guarded([&] {
         require_state(h, false);
         if (!doubles) fail(DFJK_ERR_ARG, "null output");
-        size_t ntp;
-        if (h->initialized || !h->tiles.empty()) {
-            ntp = h->ntp;
-        } else if (h->tensor_mode) {
-            const int nT = (h->nbf + TILE - 1) / TILE;
-            ntp = static_cast<size_t>(nT) * (nT + 1) / 2;
-        } else {
-            prepare_sparsity(h);
-            build_tiles(h);
-            ntp = h->ntp;
-        }
-        const size_t nbfp = static_cast<size_t>(round_up(h->nbf, TILE));
-        size_t est = static_cast<size_t>(h->naux) * ntp * TILE2;   // resident B
-        if (h->do_wK) est *= 2;                                    // and the attenuated Bw
-        est += static_cast<size_t>(h->naux) * h->naux;            // metric (fit only)
-        est += 4 * static_cast<size_t>(h->nbf) * h->nbf;          // D, J, K, staging
-        est += 2 * nbfp * 320;                                    // padded C panels
+        ensure_sparsity(h);
+        // DFHelper::AO_core (dfhelper.cc:216-236) with do_wK unset, as MemDFJK::memory_estimate
+        // calls it: the pQq tensor naux * nsig, the metric, the C buffers, the Q-shell temporaries
+        const size_t nbf = static_cast<size_t>(h->nbf), naux = static_cast<size_t>(h->naux);
+        size_t est = naux * static_cast<size_t>(h->nsig);
+        est += naux * naux;
+        est += static_cast<size_t>(h->omp_nthread) * nbf * nbf;
+        est += 3 * nbf * nbf * static_cast<size_t>(qshell_max(h));
         *doubles = est;
+    });
+}
+
+int dfjk_device_footprint(dfjk_handle* h, size_t* doubles) {
+    return guarded([&] {
+        require_state(h, false);
+        if (!doubles) fail(DFJK_ERR_ARG, "null output");
+        ensure_sparsity(h);
+        const size_t nbfp = static_cast<size_t>(round_up(h->nbf, TILE));
+        size_t est = static_cast<size_t>(h->naux) * h->ntp * TILE2;  // resident B
+        if (h->do_wK) est *= 2;                                      // and the attenuated Bw
+        est += static_cast<size_t>(h->naux) * h->naux;              // metric (fit only)
+        est += 4 * static_cast<size_t>(h->nbf) * h->nbf;            // D, J, K, staging
+        est += 2 * nbfp * 320;                                      // padded C panels
+        *doubles = est;
+    });
+}
+
+int dfjk_get_pair_mask(dfjk_handle* h, uint8_t* mask, double* cutoff) {
+    return guarded([&] {
+        require_state(h, false);
+        ensure_sparsity(h);
+        if (mask) std::memcpy(mask, h->fun_mask.data(), h->fun_mask.size());
+        if (cutoff) *cutoff = h->cutoff;
+    });
+}
+
+int dfjk_get_fit_info(dfjk_handle* h, int* fitting, double* condition, int* naux_kept) {
+    return guarded([&] {
+        require_state(h, false);
+        if (fitting) *fitting = h->fitting;
+        if (condition) *condition = h->condition;
+        if (naux_kept) *naux_kept = h->initialized && !h->tensor_mode ? h->fit_kept : 0;
+    });
+}
+
+int dfjk_set_pair_form(dfjk_handle* h, int form) {
+    return guarded([&] {
+        require_state(h, false);
+        if (form < DFJK_FORM_DETECT || form > DFJK_FORM_CROSS) fail(DFJK_ERR_ARG, "unknown pair form");
+        h->pair_form = form;
+    });
+}
+
+int dfjk_get_aux_range(dfjk_handle* h, int* q0, int* q1) {
+    return guarded([&] {
+        require_state(h, false);
+        if (q0) *q0 = h->aux_q0;
+        if (q1) *q1 = h->aux_q1;
     });
 }
 
@@ -1766,12 +2049,20 @@ int dfjk_initialize(dfjk_handle* h) {
         require_state(h, false);
         if (h->initialized) return;
         ensure_stream(h);
+        // pinned staging of the column map and the structure test, and the staging event
+        if (!h->cmap_pin) {
+            hcheck(hipHostMalloc(reinterpret_cast<void**>(&h->cmap_pin), 16384 * sizeof(int), 0), "hipHostMalloc");
+            h->cmap_pin_cap = 16384;
+        }
+        if (!h->sign_pin) {
+            hcheck(hipHostMalloc(reinterpret_cast<void**>(&h->sign_pin), 16384, 0), "hipHostMalloc");
+            h->sign_pin_cap = 16384;
+        }
+        if (!h->ev_cmap) hcheck(hipEventCreateWithFlags(&h->ev_cmap, hipEventDisableTiming), "event");
         if (h->tensor_mode) {
             if (h->subalgo == DFJK_SUBALGO_OUT_OF_CORE)
                 fail(DFJK_ERR_STATE, "DFJK: tensor-level handles are in core only");
-            h->fun_mask.assign(static_cast<size_t>(h->nbf) * h->nbf, 1);
-            h->nsig = static_cast<int64_t>(h->nbf) * h->nbf;
-            build_tiles(h);
+            ensure_sparsity(h);
             upload_tiles(h);
             h->dB.get<double>(b_doubles(h));
             hcheck(hipMemsetAsync(h->dB.p, 0, b_doubles(h) * sizeof(double), h->stream), "memset B");
@@ -1867,6 +2158,7 @@ int dfjk_set_Bw_dense(dfjk_handle* h, const double* Bw, int on_device) {
 int dfjk_fill_synthetic(dfjk_handle* h, uint64_t seed) {
     return guarded([&] {
         require_state(h, true);
+        require_device_B(h, "dfjk_fill_synthetic");
         invalidate_copies(h);
         std::vector<double> decay(h->nbf);
         for (int k = 0; k < h->nbf; ++k) decay[k] = std::exp(-static_cast<double>(k) / 64.0);
@@ -1962,6 +2254,7 @@ int dfjk_finalize(dfjk_handle* h) {
                         &h->ws_Y,    &h->ws_PC,  &h->ws_gam, &h->ws_dc, &h->ws_stage[0], &h->ws_stage[1],
                         &h->ws_stage32, &h->ws_stagebf};
         for (DevBuf* b : ws) b->release();
+        h->cmap_dev.clear();
         h->dM_ready = false;
     });
 }

@@ -497,6 +497,28 @@ __global__ void k_fill_scaled(const double* __restrict__ d, int q0, double a, co
         G[q * nn + i] = f * D[i];
 }
 
+// p[0..n) = 0 (a kernel, not hipMemsetAsync: a caller may capture this work in a graph)
+__global__ void k_zero(double* __restrict__ p, size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        p[i] = 0.0;
+}
+void launch_zero(double* p, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    const size_t nb = std::min<size_t>((n + 255) / 256, size_t(1) << 16);
+    hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, p, n);
+}
+
+// out[r][c] = f[r] * A[r][c]
+__global__ void k_scale_rows(const double* __restrict__ A, int rows, int cols, const double* __restrict__ f,
+                             double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (c < cols) out[static_cast<size_t>(r) * cols + c] = f[r] * A[static_cast<size_t>(r) * cols + c];
+}
+void launch_scale_rows(const double* A, int rows, int cols, const double* f, double* out, hipStream_t s) {
+    require(rows >= 1 && rows <= 65535 && cols >= 1, "scale_rows");
+    hipLaunchKernelGGL(k_scale_rows, dim3((cols + 255) / 256, rows), dim3(256), 0, s, A, rows, cols, f, out);
+}
 void launch_axpby_mat(const double* X, double a, const double* Y, double b, double* Z, size_t n, hipStream_t s) {
     const size_t nb = std::min<size_t>((n + 255) / 256, size_t(1) << 20);
     if (nb == 0) return;

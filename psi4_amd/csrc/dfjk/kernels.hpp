@@ -71,6 +71,10 @@ void launch_axpby_mat(const double* X, double a, const double* Y, double b, doub
 void launch_hermitivitize(double* A, int n, int lda, hipStream_t s);
 // partial[0..nblocks) = block sums of X.*Y (deterministic; sum the partials on the host)
 void launch_dot_partial(const double* X, const double* Y, size_t n, double* partial, int nblocks, hipStream_t s);
+// p[0..n) = 0 by a kernel (graph-capturable, unlike hipMemsetAsync)
+void launch_zero(double* p, size_t n, hipStream_t s);
+// out[r][c] = f[r] * A[r][c] (rows x cols, dense row-major)
+void launch_scale_rows(const double* A, int rows, int cols, const double* f, double* out, hipStream_t s);
 // AT[c][r] = A[r][c]
 void launch_transpose(const double* A, int rows, int cols, int lda, double* AT, int ldt, hipStream_t s);
 // G[q][:][:] = a * d[q0+q] * D for q < nq (D n x n)

@@ -82,6 +82,30 @@ class JK {
         check(dfjk_memory_estimate(h_, &d));
         return d;
     }
+    size_t device_footprint() {
+        size_t d = 0;
+        check(dfjk_device_footprint(h_, &d));
+        return d;
+    }
+    // Schwarz function-pair mask (nbf x nbf, 0/1) of the engine's screening
+    py::array_t<uint8_t> pair_mask() {
+        py::array_t<uint8_t> m({static_cast<py::ssize_t>(nbf_), static_cast<py::ssize_t>(nbf_)});
+        check(dfjk_get_pair_mask(h_, m.mutable_data(), nullptr));
+        return m;
+    }
+    // declared structure of non-symmetric pairs: "DETECT", "GENERAL" or "CROSS"
+    void set_pair_form(const std::string& f) {
+        int v;
+        if (f == "DETECT")
+            v = DFJK_FORM_DETECT;
+        else if (f == "GENERAL")
+            v = DFJK_FORM_GENERAL;
+        else if (f == "CROSS")
+            v = DFJK_FORM_CROSS;
+        else
+            throw std::runtime_error("JK: pair form must be DETECT, GENERAL or CROSS");
+        check(dfjk_set_pair_form(h_, v));
+    }
     void initialize() {
         check(dfjk_initialize(h_));
         initialized_ = true;
@@ -295,6 +319,17 @@ class JK {
         const char* names[6] = {"screening", "metric", "metric_factor", "three_index", "fit", "attenuated"};
         for (int i = 0; i < 6; ++i) t[names[i]] = ms[i];
         d["setup_ms"] = t;
+        int fitting = 0, kept = 0;
+        double cond = 0.0, cutoff = 0.0;
+        check(dfjk_get_fit_info(h_, &fitting, &cond, &kept));
+        check(dfjk_get_pair_mask(h_, nullptr, &cutoff));
+        d["fitting"] = fitting;
+        d["condition"] = cond;
+        d["naux_kept"] = kept;
+        d["cutoff"] = cutoff;
+        int q0 = 0, q1 = 0;
+        check(dfjk_get_aux_range(h_, &q0, &q1));
+        d["aux_range"] = py::make_tuple(q0, q1);
         return d;
     }
 
@@ -379,12 +414,15 @@ PYBIND11_MODULE(_core, m) {
     m.attr("PREC_BF16X3") = static_cast<int>(DFJK_PREC_BF16X3);
     m.attr("FIT_CHOLESKY") = static_cast<int>(DFJK_FIT_CHOLESKY);
     m.attr("FIT_EIGEN") = static_cast<int>(DFJK_FIT_EIGEN);
-    py::class_<JK, std::shared_ptr<JK>>(m, "JK")
+    py::class_<JK, std::shared_ptr<JK>>(m, "JK", py::dynamic_attr())
         .def_static("build_JK", &JK::build, py::arg("basis"), py::arg("aux"), py::arg("do_wK") = false,
                     py::arg("doubles") = static_cast<size_t>(256000000))
         .def_static("build_tensor", &JK::build_tensor, py::arg("nbf"), py::arg("naux"))
         .def("name", &JK::name)
         .def("memory_estimate", &JK::memory_estimate)
+        .def("device_footprint", &JK::device_footprint)
+        .def("pair_mask", &JK::pair_mask)
+        .def("set_pair_form", &JK::set_pair_form)
         .def("initialize", &JK::initialize)
         .def("set_print", &JK::set_print)
         .def("set_cutoff", &JK::set_cutoff)

@@ -8,7 +8,8 @@ for a closed-shell RHF on the in-core DF JK:
   Overlap    -sum W_t dS/dR, W_t = 2 C_occ eps_occ C_occ^T    (scf_grad.cc:194-221)
   J and K    DFJKGrad (jk_grad.cc:175-293, 834-1000), here from the adjoint of
              E_2 = tr(D(2J - K)) with respect to the fitted tensor B = M (P|mn),
-             M^T M = (P|Q)^-1 (M = L^-1 of the host Cholesky factor):
+             M^T M = (P|Q)^-1 (M = L^-1 of the Cholesky factor, or the symmetric
+             J^-1/2 of psi4's eigen fit, whichever the JK object used):
                dE_2/d(P|mn) = (M^T Gamma)_Pmn = 4 d_P D_mn - 2 (D A~_P D)_mn
                dE_2/d(P|Q)  = -2 d d^T + M^T W M,   W_QR = <C^T B_Q C, C^T B_R C>,
              with d = M^T d', d'_Q = <B_Q, D> -- the quantities psi4 forms as c_A, d_A
@@ -37,15 +38,54 @@ def _schwarz_pair_mask(basis, cutoff=1e-12):
     return fm >= cutoff * cutoff / mx
 
 
+def _pair_mask(jk, basis):
+    """The engine's own Schwarz mask (its cutoff), or the default rule for a JK without one."""
+    if hasattr(jk, "pair_mask"):
+        return jk.pair_mask().astype(bool)
+    return _schwarz_pair_mask(basis)
+
+
 def _device_adjoint(jk):
     naux = jk.info()["naux"]
-    jk.set_aux_range(0, naux)
     B = jk.get_B_dense(0, naux)
 
     def adj(Co):
-        E2, _, Gam = jk.energy_adjoint(np.ascontiguousarray(Co), True)
+        q0, q1 = jk.info()["aux_range"]
+        jk.set_aux_range(0, naux)      # Gamma over every Q; the caller's range is restored after
+        try:
+            E2, _, Gam = jk.energy_adjoint(np.ascontiguousarray(Co), True)
+        finally:
+            jk.set_aux_range(q0, q1)
         return E2, Gam, B
     return adj
+
+
+class _MetricFactor:
+    """The factor M of the fit B = M (P|mn) the JK object used: L^-1 of the Cholesky factor
+    J = L L^T, or psi4's eigen J^-1/2 with the condition cut (libmints/matrix.cc:2370-2424).
+    apply_T(X) returns M^T X for X [naux, ...]."""
+
+    def __init__(self, aux, jk=None):
+        J = ints.metric(aux)
+        info = jk.info() if jk is not None and hasattr(jk, "info") else {}
+        if info.get("fitting", 0) == 1:
+            a, V = np.linalg.eigh(J)
+            max_a = max(abs(a[-1]), abs(a[0]))
+            with np.errstate(divide="ignore", invalid="ignore"):
+                f = np.where(np.abs(a) < info["condition"] * max_a, 0.0, a ** -0.5)
+            f[~np.isfinite(f)] = 0.0
+            self.M = (V * f) @ V.T
+            self.L = None
+        else:
+            self.L = cholesky(J, lower=True)
+            self.M = None
+
+    def apply_T(self, X):
+        shape = X.shape
+        X2 = X.reshape(shape[0], -1)
+        if self.L is not None:
+            return solve_triangular(self.L, X2, lower=True, trans="T").reshape(shape)
+        return (self.M.T @ X2).reshape(shape)
 
 
 def _one_electron_terms(mol, bs, Dt, Wt):
@@ -57,17 +97,16 @@ def _one_electron_terms(mol, bs, Dt, Wt):
             "Overlap": -_per_atom(ints.overlap_grad(bs, Wt), bs, natom)}
 
 
-def _jk_term(mol, bs, aux, Gam, coef_fit, L, nthreads):
+def _jk_term(mol, bs, aux, Gam, coef_fit, Mf, mask, nthreads):
     """Contract dE/dB (Gamma, fitted basis) and dE/dJ (given in the fitted basis as
     M^-T (dE/dJ) M^-1 = coef_fit) with the derivative integrals."""
-    naux, nbf = Gam.shape[0], Gam.shape[1]
-    G = solve_triangular(L, Gam.reshape(naux, -1), lower=True, trans="T").reshape(naux, nbf, nbf)
-    G *= _schwarz_pair_mask(bs)[None, :, :]
+    G = Mf.apply_T(Gam)
+    G *= mask[None, :, :]
     gp, ga = ints.three_index_grad(bs, aux, G, nthreads)
     del G
-    # dE/dJ = M^T coef_fit M with M = L^-1
-    T = solve_triangular(L, coef_fit, lower=True, trans="T")
-    coef = solve_triangular(L, T.T, lower=True, trans="T").T
+    # dE/dJ = M^T coef_fit M
+    T = Mf.apply_T(coef_fit)
+    coef = Mf.apply_T(np.ascontiguousarray(T.T)).T
     coef = 0.5 * (coef + coef.T)
     ga = ga + ints.metric_grad(aux, coef, nthreads)
     return _per_atom(gp, bs, mol.natom) + _per_atom(ga, aux, mol.natom)
@@ -93,12 +132,13 @@ def rhf_gradient(wfn, adjoint=None, nthreads=None):
     Co = np.ascontiguousarray(wfn.Ca[:, :wfn.nocc])
     eps = wfn.eps[:wfn.nocc]
     terms = _one_electron_terms(mol, bs, 2.0 * Co @ Co.T, 2.0 * (Co * eps) @ Co.T)
-    adjoint = adjoint or _device_adjoint(wfn.jk())
+    jk = wfn.jk() if adjoint is None else getattr(wfn, "_jk", None)
+    adjoint = adjoint or _device_adjoint(jk)
     E2, Gam, B = adjoint(Co)
-    L = cholesky(ints.metric(aux), lower=True)
+    Mf = _MetricFactor(aux, jk)
     # dE/d(P|Q) = -2 d d^T + M^T W M  (fitted: -2 d' d'^T + W)
     dq, Wq = _fitted_d_and_w(B, Co)
-    terms["JK"] = _jk_term(mol, bs, aux, Gam, -2.0 * np.outer(dq, dq) + Wq, L, nthreads)
+    terms["JK"] = _jk_term(mol, bs, aux, Gam, -2.0 * np.outer(dq, dq) + Wq, Mf, _pair_mask(jk, bs), nthreads)
     terms["E_2"] = E2
     g = terms["Nuclear"] + terms["Core"] + terms["Overlap"] + terms["JK"]
     return g, terms
@@ -121,7 +161,8 @@ def uhf_gradient(wfn, adjoint=None, nthreads=None):
     Da, Db = Ca @ Ca.T, Cb @ Cb.T
     Wt = (Ca * wfn.eps_a[:wfn.nalpha]) @ Ca.T + (Cb * wfn.eps_b[:wfn.nbeta]) @ Cb.T
     terms = _one_electron_terms(mol, bs, Da + Db, Wt)
-    adjoint = adjoint or _device_adjoint(wfn.jk())
+    jk = wfn.jk() if adjoint is None else getattr(wfn, "_jk", None)
+    adjoint = adjoint or _device_adjoint(jk)
     Ea, Ga, B = adjoint(Ca)
     Eb, Gb, _ = adjoint(Cb)
     dqa, Wa = _fitted_d_and_w(B, Ca)
@@ -130,9 +171,10 @@ def uhf_gradient(wfn, adjoint=None, nthreads=None):
     dqs = dqa - dqb
     Gam = 0.5 * (Ga + Gb) - dqs[:, None, None] * Ds[None, :, :]
     del Ga, Gb
-    L = cholesky(ints.metric(aux), lower=True)
+    Mf = _MetricFactor(aux, jk)
     dqt = dqa + dqb
-    terms["JK"] = _jk_term(mol, bs, aux, Gam, -0.5 * np.outer(dqt, dqt) + 0.5 * (Wa + Wb), L, nthreads)
+    terms["JK"] = _jk_term(mol, bs, aux, Gam, -0.5 * np.outer(dqt, dqt) + 0.5 * (Wa + Wb), Mf,
+                           _pair_mask(jk, bs), nthreads)
     terms["E_2"] = 0.5 * (Ea + Eb) - 0.5 * float(dqs @ dqs)
     g = terms["Nuclear"] + terms["Core"] + terms["Overlap"] + terms["JK"]
     return g, terms

@@ -54,6 +54,29 @@ def _copy(a):
     return a.clone() if torch.is_tensor(a) else a.copy()
 
 
+def orthogonalizer(S, s_tolerance=1e-7):
+    """X with X^T S X = 1: psi4's S_ORTHOGONALIZATION AUTO (libmints/orthog.cc:58-77, 176-235,
+    330-360; S_TOLERANCE default 1e-7, read_options.cc:1497).  S is normalised to unit
+    diagonal, Sn = U s U^T; if min(s) >= s_tolerance the symmetric X = U s^-1/2 U^T is used,
+    else the canonical X = U_k s_k^-1/2 over the eigenvalues above the tolerance (columns in
+    increasing eigenvalue order, nmo < nbf); the rows are then scaled back by the
+    normalisation.  psi4's third choice, partial Cholesky for a pathologically overcomplete
+    basis (reciprocal condition <= machine epsilon), is not provided: that case raises."""
+    n = 1.0 / torch.sqrt(torch.diagonal(S))
+    Sn = S * n[:, None] * n[None, :]
+    s, U = torch.linalg.eigh(Sn)
+    rcond = float(s[0] / s[-1])
+    if not (rcond > torch.finfo(S.dtype).eps and np.isfinite(rcond)):
+        raise RuntimeError("overlap matrix is pathologically overcomplete (partial Cholesky "
+                           "orthogonalization is not provided)")
+    if float(s[0]) < s_tolerance:
+        keep = s > s_tolerance
+        X = U[:, keep] * s[keep].rsqrt()
+    else:
+        X = (U * s.rsqrt()) @ U.T
+    return X * n[:, None]
+
+
 class _DIIS:
     """Pulay DIIS over numpy arrays or torch tensors (host or device)."""
 
@@ -98,7 +121,7 @@ class RHF:
 
     def __init__(self, mol, basis="cc-pvdz", df_basis_scf=None, jk=None, e_convergence=1e-10, d_convergence=1e-8,
                  maxiter=100, diis_max_vecs=10, mixed_precision=None, mixed_until=1e-7, incfock_full_every=8,
-                 fail_on_maxiter=True, jk_callback=None, perturb_dipole=None):
+                 fail_on_maxiter=True, jk_callback=None, perturb_dipole=None, s_tolerance=1e-7):
         from .basisset import BasisSet
         self.mol = mol
         self.basis = BasisSet.build(mol, basis)
@@ -115,6 +138,7 @@ class RHF:
         self.full_every = max(1, int(incfock_full_every))
         self.fail_on_maxiter = fail_on_maxiter
         self.jk_callback = jk_callback  # called as cb(iteration, C_occ, J, K, kind) after every build
+        self.s_tolerance = s_tolerance
         self._jk = jk
         self._injected = jk is not None
         self.Ca = self.Fa = self.eps = None
@@ -157,10 +181,14 @@ class RHF:
             return torch.device("cuda", torch.cuda.current_device())
         return torch.device("cpu")
 
-    def _jk_call(self, prec, Cl, Cr=None):
-        """J, K of one density pair (C_left, C_right) as tensors on the SCF device."""
+    def _jk_call(self, prec, Cl, Cr=None, form=None):
+        """J, K of one density pair (C_left, C_right) as tensors on the SCF device.  form
+        declares the pair's structure to the engine ("CROSS"), which then skips its test."""
         jk = self.jk()
         jk.set_precision(prec)
+        declared = form is not None and hasattr(jk, "set_pair_form")
+        if declared:
+            jk.set_pair_form(form)
         try:
             if Cl.is_cuda:
                 nbf, ncol = Cl.shape
@@ -179,6 +207,8 @@ class RHF:
             return torch.from_numpy(jk.J()[0].copy()), torch.from_numpy(jk.K()[0].copy())
         finally:
             jk.set_precision(PREC_FP64)
+            if declared:
+                jk.set_pair_form("DETECT")
 
     def _full_build(self, Cocc):
         return self._jk_call(PREC_FP64, Cocc)
@@ -194,7 +224,7 @@ class RHF:
         V = Cnew - Col
         A = 0.5 * (Cnew + Col)
         return self._jk_call(_PREC[self.mixed], torch.cat([A, V], 1).contiguous(),
-                             torch.cat([V, A], 1).contiguous())
+                             torch.cat([V, A], 1).contiguous(), form="CROSS")
 
     def compute_energy(self):
         t0 = time.perf_counter()
@@ -207,10 +237,11 @@ class RHF:
         dev = self._device()
         S = torch.from_numpy(S_h).to(dev)
         H = torch.from_numpy(H_h).to(dev)
-        # symmetric orthogonalizer X = S^-1/2; F C = S C eps is solved as (X^T F X) C' = C' eps,
-        # C = X C' (hf.cc form_C with the symmetric orthogonalization)
-        w, U = torch.linalg.eigh(S)
-        X = (U * w.rsqrt()) @ U.T
+        # orthogonalizer X (symmetric, or canonical when S is near-singular); F C = S C eps is
+        # solved as (X^T F X) C' = C' eps, C = X C' (hf.cc form_Shalf / form_C)
+        X = orthogonalizer(S, self.s_tolerance)
+        if X.shape[1] < self.nocc:
+            raise RuntimeError("Not enough molecular orbitals to satisfy requested occupancies")
 
         def diag(F):
             e, Cp = torch.linalg.eigh(X.T @ F @ X)

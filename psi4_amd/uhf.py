@@ -15,7 +15,7 @@ import torch
 
 from . import ints
 from .core import JK, PREC_FP64
-from .scf import SCFConvergenceError, _DIIS
+from .scf import SCFConvergenceError, _DIIS, orthogonalizer
 
 
 class UHFResult:
@@ -25,7 +25,7 @@ class UHFResult:
 
 class UHF:
     def __init__(self, mol, basis="cc-pvdz", df_basis_scf=None, jk=None, e_convergence=1e-10, d_convergence=1e-8,
-                 maxiter=100, diis_max_vecs=10, guess=None):
+                 maxiter=100, diis_max_vecs=10, guess=None, s_tolerance=1e-7):
         from .basisset import BasisSet
         self.mol = mol
         self.basis = BasisSet.build(mol, basis)
@@ -39,6 +39,7 @@ class UHF:
         self.e_conv, self.d_conv, self.maxiter = e_convergence, d_convergence, maxiter
         self.diis_max_vecs = diis_max_vecs
         self.guess = guess  # None: core Hamiltonian; or starting orbitals (nbf x nmo) for both spins
+        self.s_tolerance = s_tolerance
         self._jk = jk
         self.Ca = self.Cb = self.Fa = self.Fb = self.eps_a = self.eps_b = None
 
@@ -82,8 +83,9 @@ class UHF:
         S = torch.from_numpy(ints.overlap(bs)).to(dev)
         H = torch.from_numpy(ints.kinetic(bs) + ints.potential(bs, mol)).to(dev)
         enuc = mol.nuclear_repulsion_energy()
-        w, U = torch.linalg.eigh(S)
-        X = (U * w.rsqrt()) @ U.T
+        X = orthogonalizer(S, self.s_tolerance)
+        if X.shape[1] < max(self.nalpha, self.nbeta):
+            raise RuntimeError("Not enough molecular orbitals to satisfy requested occupancies")
 
         def diag(F):
             e, Cp = torch.linalg.eigh(X.T @ F @ X)

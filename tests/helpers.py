@@ -70,3 +70,20 @@ class OracleJK:
 
     def K(self):
         return self._K
+
+
+def per_build_checker(jk, worst):
+    """jk_callback that checks every mixed incremental build against a plain FP64 build of the
+    same orbitals: |E_build(mixed) - E_build(FP64)| with E_build = tr(D(2J - K))."""
+    def cb(it, C, J, K, kind):
+        if kind == "fp64":
+            return
+        jk.C_clear()
+        jk.C_left_add(C)
+        jk.compute()
+        D = C @ C.T
+        e_mixed = float(np.sum(D * (2.0 * J - K)))
+        e_ref = float(np.sum(D * (2.0 * jk.J()[0] - jk.K()[0])))
+        worst.append(abs(e_mixed - e_ref))
+        print(f"  iteration {it} ({kind}): |dE_build| {worst[-1]:.2e}", flush=True)
+    return cb

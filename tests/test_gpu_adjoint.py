@@ -135,7 +135,31 @@ def test_autograd_vs_central_differences_h2o16(h2o16):
         assert err < 2e-6, (q, m, k, fd, an)
         done += 1
     print(f"  worst B central-difference error {worst:.2e}", flush=True)
-    jk.set_B_dense_device(Bt.detach().data_ptr())
+
+
+def test_autograd_leaves_the_jk_unchanged(h2o16):
+    """scf_energy_2e with a caller's B (forward and backward) does not touch the JK object:
+    a compute() after it equals one before it, bit for bit, and its aux range is kept."""
+    import torch
+    from psi4_amd.autograd import scf_energy_2e
+    jk, B, C, _ = h2o16
+    dev = torch.device("cuda:0")
+    announce("autograd side effects: compute() before and after scf_energy_2e(jk, C, B')")
+    naux = B.shape[0]
+    jk.C_clear()
+    jk.C_left_add(C)
+    jk.compute()
+    J0, K0 = jk.J()[0].copy(), jk.K()[0].copy()
+    jk.set_aux_range(10, naux - 10)
+    Bp = torch.from_numpy(1.01 * B).to(dev).requires_grad_(True)
+    Ct = torch.from_numpy(C).to(dev).requires_grad_(True)
+    E = scf_energy_2e(jk, Ct, Bp)
+    E.backward()
+    assert jk.info()["aux_range"] == (10, naux - 10)
+    jk.set_aux_range(0, naux)
+    jk.compute()
+    assert np.array_equal(jk.J()[0], J0) and np.array_equal(jk.K()[0], K0)
+    assert np.array_equal(jk.get_B_dense(0, 4), B[:4])
 
 
 @pytest.mark.parametrize("nocc", [72, 330])
@@ -149,7 +173,7 @@ def test_adjoint_tensor_blocks_and_panels(gpu_available, nocc):
     jk = JK.build_tensor(nbf, naux)
     jk.initialize()
     jk.fill_synthetic(77)
-    est = jk.memory_estimate()
+    est = jk.device_footprint()
     jk.set_memory(int(est + 16 * (256 * 384 + nocc * nocc + nbf * nocc + nbf * nbf)))
     B = jk.get_B_dense(0, naux)
     C = np.random.default_rng(nocc).standard_normal((nbf, nocc)) / np.sqrt(nbf)

@@ -85,3 +85,21 @@ def test_out_of_core_matches_in_core(gpu_available):
     assert _rel(Jo, Jc) < 1e-12 and _rel(Ko, Kc) < 1e-6
     with pytest.raises(RuntimeError, match="in core"):
         ooc.energy_adjoint(C)
+
+
+def test_out_of_core_rejects_device_tensor_paths(gpu_available):
+    """B streamed from host memory: wK (psi4 refuses it out of core, dfhelper.cc:201-205) and
+    the synthetic fill, which both need B on the device, fail with a status before any
+    kernel is launched; J/K still run afterwards."""
+    from psi4_amd.molecule import water_cluster
+    mol = water_cluster((2, 2, 1), seed=0)
+    announce("(H2O)_4 out of core: wK and the synthetic fill are refused")
+    jk = _jk(mol, subalgo="OUT_OF_CORE")
+    assert jk.info()["out_of_core"]
+    with pytest.raises(RuntimeError, match="out of core"):
+        jk.set_do_wK(True)
+    with pytest.raises(RuntimeError, match="resident on the device"):
+        jk.fill_synthetic(3)
+    C = np.random.default_rng(2).standard_normal((96, 20)) / 10.0
+    J, K = _run(jk, C)
+    assert np.isfinite(J).all() and np.isfinite(K).all()
