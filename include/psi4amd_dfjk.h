@@ -157,6 +157,20 @@ int dfjk_compute(dfjk_handle* h, int n, const double* const* Cl, const double* c
                  double* const* D, double* const* J, double* const* K, int accumulate);
 int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
                         double* const* D, double* const* J, double* const* K, int accumulate);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): dfjk_compute_device
+ * with the same arguments and results, but every workspace buffer comes from one static
+ * 128 MiB device array per GPU, held from the library's load until the process ends; no
+ * hipMalloc, hipFree or allocating library routine is called.  Limits: B in core; for
+ * DFJK_PREC_FP32 / DFJK_PREC_BF16X3 one dfjk_compute_device at that precision must have made
+ * the lower-precision copy of B first; a system whose fixed-size buffers (D, J partials, one
+ * K split, one auxiliary function's half-transform) exceed the array is refused with
+ * DFJK_ERR_MEMORY before any work is queued (from about nbf 1150 at 300 columns).  Every handle on
+ * a GPU shares that GPU's array, which keeps the last call's contents: calls that overlap on
+ * one GPU (other threads, other streams) race. */
+int dfjk_compute_device_noalloc(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr,
+                                const int* ncol, double* const* D, double* const* J, double* const* K,
+                                int accumulate);
 /* JK::wK() of JK::compute with do_wK (jk.cc:382-423) -> DFHelper::compute_wK
  * (dfhelper.cc:3378-3438), MemDFJK's non-combined path (wcombine off, MemDFJK.cc:81-90):
  *   wK = sum_Q (B_Q Cl)(Bw_Q Cr)^T = sum_{PQ} ((P|mn) Cl) [J^-1]_PQ ((Q|erf(omega r)/r|mn) Cr)^T,

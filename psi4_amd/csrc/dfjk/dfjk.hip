@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -27,6 +28,13 @@
 #include "kernels_ints.hpp"
 #include "kernels_k.hpp"
 #include "psi4amd_dfjk.h"
+
+// Workspace of dfjk_compute_device_noalloc: one array per GPU, held from the library's load
+// until the process ends.  The name below is a host-side stand-in, never a device address:
+// host code uses only the address hipGetSymbolAddress returns for each GPU (see
+// scratch_of_device), kept per device.  Every handle on a GPU shares that GPU's copy.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
 
 namespace p4a {
 namespace {
@@ -161,6 +169,8 @@ struct dfjk_handle {
     int fit_kept = 0;       // metric eigenvalues kept by the fit (naux for Cholesky)
     bool initialized = false;
     int aux_q0 = 0, aux_q1 = 0;
+    int device = -1;              // the HIP device current at initialize (its buffers live there)
+    char* scratch_dev = nullptr;  // that device's address of the static scratch array
     // stream
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
@@ -962,6 +972,23 @@ void invalidate_copies(dfjk_handle* h) {
     h->dBlT.release();
 }
 
+// Device address of the scratch array on the current device, looked up once per device with
+// that device current (each GPU holds its own copy at its own address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_addr[kMaxDevices] = {};
+char* scratch_of_device(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) fail(DFJK_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        hcheck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) fail(DFJK_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_addr[dev] = static_cast<char*>(p);
+    }
+    return g_scratch_addr[dev];
+}
+
 // Device workspace of one compute.  Allocating mode: the handle's growable buffers.  Fixed
 // mode (dfjk_compute_device_noalloc): a bump arena over this device's static scratch array;
 // nothing is allocated, and a request past the arena's end is an error.
@@ -1039,6 +1066,27 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     const int xrows = (h->nT + 3) / 4 * 128;
     EventTimer& tm = h->timer;
     if (wk.fixed() && h->ooc) fail(DFJK_ERR_STATE, "dfjk_compute_device_noalloc needs B in core");
+    if (wk.fixed()) {
+        // Before any launch: the buffers of fixed size, one K split and one auxiliary
+        // function's X must fit the scratch array (X blocks and splits then adapt to the rest).
+        auto r = [](size_t b) { return (std::max<size_t>(b, 256) + 255) / 256 * 256; };
+        int nbw_b, np_b, ldc_b;
+        ht_panel_shape(ncol + 64, &nbw_b, &np_b, &ldc_b);  // column groups are padded to 32 each
+        const size_t ldcb = static_cast<size_t>(ldc_b) + 320;
+        const size_t qsb = static_cast<size_t>(h->ntp) * TILE2, cszb = static_cast<size_t>(nbfp) * ldcb;
+        const size_t tiles = static_cast<size_t>(h->ntile_all) * 16384 * sizeof(double);
+        const size_t need = r(2 * static_cast<size_t>(ncol)) + r(ldcb * sizeof(int)) + 2 * r(cszb * 8) +
+                            r(static_cast<size_t>(nbf) * nbf * 8) + r(tiles) +                       // D
+                            r(qsb * 8) + r(static_cast<size_t>(h->naux) * 8) +                       // J: W, d
+                            r(static_cast<size_t>(h->ntp) * h->naux * 8) + r(qsb * 8) +             // J partials
+                            r(tiles) + r(8 * cszb) +                                                  // K split, C copies
+                            2 * r(static_cast<size_t>(xrows) * ldcb * 8) + 1024;                     // one Q of X
+        if (need > wk.cap)
+            fail(DFJK_ERR_MEMORY, "dfjk_compute_device_noalloc: nbf " + std::to_string(nbf) + ", naux " +
+                                      std::to_string(h->naux) + " and " + std::to_string(ncol) +
+                                      " columns need about " + std::to_string(need >> 20) + " MiB, beyond the " +
+                                      std::to_string(wk.cap >> 20) + " MiB scratch array");
+    }
 
     // classify the pair (or take the declared form) and build the column map
     bool sym = (dCr == nullptr), cross = false;
@@ -1540,15 +1588,15 @@ void reset_timers(dfjk_handle* h) {
     ensure_stream(h);
 }
 
-// scratch: null for the allocating entry points, else this device's static scratch array
-// (dfjk_compute_device_noalloc; device pointers only, no timing events)
+// arena: null for the allocating entry points, else the device address of this device's
+// static scratch array (dfjk_compute_device_noalloc; device pointers only, no timing events)
 void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
                   double* const* D, double* const* J, double* const* K, int accumulate, bool device,
-                  char* scratch = nullptr, size_t scratch_bytes = 0) {
+                  char* arena = nullptr, size_t arena_bytes = 0) {
     require_state(h, true);
     if (n < 0 || (n > 0 && (!Cl || !ncol))) fail(DFJK_ERR_ARG, "JK: bad C_left arguments");
     const bool timing = h->timer.on;
-    if (scratch) h->timer.on = false;
+    if (arena) h->timer.on = false;
     struct TimerRestore {
         dfjk_handle* h;
         bool on;
@@ -1609,8 +1657,8 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
             }
         }
         Work wk;  // per pair: the pairs run in stream order, so a fixed arena is reused
-        wk.base = scratch;
-        wk.cap = scratch_bytes;
+        wk.base = arena;
+        wk.cap = arena_bytes;
         compute_pair(h, cl, cr, ncol[i], gD, gJ, gK, accumulate, wk);
         if (!device) {
             if (dD) hcheck(hipMemcpyAsync(dD, gD, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download D");
@@ -2059,6 +2107,9 @@ int dfjk_initialize(dfjk_handle* h) {
             h->sign_pin_cap = 16384;
         }
         if (!h->ev_cmap) hcheck(hipEventCreateWithFlags(&h->ev_cmap, hipEventDisableTiming), "event");
+        // the device this handle's buffers live on, and its copy of the scratch array
+        hcheck(hipGetDevice(&h->device), "hipGetDevice");
+        h->scratch_dev = scratch_of_device(h->device);
         if (h->tensor_mode) {
             if (h->subalgo == DFJK_SUBALGO_OUT_OF_CORE)
                 fail(DFJK_ERR_STATE, "DFJK: tensor-level handles are in core only");
@@ -2193,6 +2244,15 @@ int dfjk_compute(dfjk_handle* h, int n, const double* const* Cl, const double* c
 int dfjk_compute_device(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,
                         double* const* D, double* const* J, double* const* K, int accumulate) {
     return guarded([&] { compute_impl(h, n, Cl, Cr, ncol, D, J, K, accumulate, true); });
+}
+
+int dfjk_compute_device_noalloc(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr,
+                                const int* ncol, double* const* D, double* const* J, double* const* K, int accumulate) {
+    return guarded([&] {
+        require_state(h, true);
+        if (!h->scratch_dev) fail(DFJK_ERR_STATE, "dfjk_compute_device_noalloc: no scratch address for this device");
+        compute_impl(h, n, Cl, Cr, ncol, D, J, K, accumulate, true, h->scratch_dev, kScratchBytes);
+    });
 }
 
 int dfjk_compute_wK(dfjk_handle* h, int n, const double* const* Cl, const double* const* Cr, const int* ncol,

@@ -301,6 +301,8 @@ class JK {
         d["n_kgemm"] = nkg;
         return d;
     }
+    // the C-ABI handle behind this object, for direct calls into libpsi4amd_dfjk
+    uintptr_t handle() const { return reinterpret_cast<uintptr_t>(h_); }
     py::dict info() {
         int nbf, naux, ntp;
         int64_t nsig;
@@ -472,5 +474,6 @@ PYBIND11_MODULE(_core, m) {
         .def("get_Bw_dense", &JK::get_Bw_dense)
         .def("timings", &JK::timings)
         .def("info", &JK::info)
+        .def("handle", &JK::handle)
         .def("print_header", &JK::print_header);
 }
