@@ -82,8 +82,10 @@ def memory_estimate(sp: Sparsity, qshell_max: int, nthreads: int = 1, do_wK=Fals
 def metric_power(J, power=-0.5, condition=1e-10):
     """libmints/matrix.cc:2370-2424 (Matrix::power): eigendecomposition, eigenvalues
     with |a| < condition*max|a| dropped for negative powers, non-finite results
-    zeroed.  Returns (J^power, number kept, eigenvalues)."""
-    a, V = np.linalg.eigh(J)
+    zeroed.  The eigendecomposition is LAPACK DSYEV, the routine psi4 calls (C_DSYEV,
+    matrix.cc:2391-2393).  Returns (J^power, number kept, eigenvalues)."""
+    from scipy.linalg import eigh
+    a, V = eigh(J, driver="ev")
     max_a = max(abs(a[-1]), abs(a[0]))
     ap = np.empty_like(a)
     remain = 0

@@ -39,26 +39,58 @@ __global__ void k_pack_w(const double* __restrict__ D, int ldD, int nbf, const i
     }
 }
 
-// One wave per Q: the wave holds the W tile (16 doubles per lane) and reduces
-// <B_Q tile, W tile> with a butterfly.  grid = (ntp, ceil((q1-q0)/qblk)), 256 threads.
 // Element loaders of the resident tensor for the J path (FP64 accumulation in every case).
+// load4 / load8 read 4 / 8 consecutive elements (i a multiple of 4 / 8) with 16-byte (or, for
+// the BF16 pair's 4-element reads, 8-byte) vector loads per lane.
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef float f4v_t __attribute__((ext_vector_type(4)));
 struct BSrcF64 {
     const double* p;
-    __device__ double operator()(size_t i) const { return p[i]; }
+    __device__ void load4(size_t i, double* v) const {
+        const d2_t a = *reinterpret_cast<const d2_t*>(p + i), b = *reinterpret_cast<const d2_t*>(p + i + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+    __device__ void load8(size_t i, double* v) const {
+        load4(i, v);
+        load4(i + 4, v + 4);
+    }
 };
 struct BSrcF32 {
     const float* p;
-    __device__ double operator()(size_t i) const { return static_cast<double>(p[i]); }
+    __device__ void load4(size_t i, double* v) const {
+        const f4v_t a = *reinterpret_cast<const f4v_t*>(p + i);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+    __device__ void load8(size_t i, double* v) const {
+        load4(i, v);
+        load4(i + 4, v + 4);
+    }
 };
 struct BSrcBF2 {  // B ~ hi + lo, two BF16 copies
     const uint16_t* h;
     const uint16_t* l;
-    __device__ static double f(uint16_t v) { return static_cast<double>(__uint_as_float(static_cast<uint32_t>(v) << 16)); }
-    __device__ double operator()(size_t i) const { return f(h[i]) + f(l[i]); }
+    __device__ static double f(uint32_t bits) { return static_cast<double>(__uint_as_float(bits)); }
+    __device__ static void add(uint32_t hw, uint32_t lw, double* v) {  // two packed BF16 of each copy
+        v[0] = f(hw << 16) + f(lw << 16);
+        v[1] = f(hw & 0xFFFF0000u) + f(lw & 0xFFFF0000u);
+    }
+    __device__ void load4(size_t i, double* v) const {
+        const uint2 a = *reinterpret_cast<const uint2*>(h + i), b = *reinterpret_cast<const uint2*>(l + i);
+        add(a.x, b.x, v);
+        add(a.y, b.y, v + 2);
+    }
+    __device__ void load8(size_t i, double* v) const {
+        const uint4 a = *reinterpret_cast<const uint4*>(h + i), b = *reinterpret_cast<const uint4*>(l + i);
+        add(a.x, b.x, v);
+        add(a.y, b.y, v + 2);
+        add(a.z, b.z, v + 4);
+        add(a.w, b.w, v + 6);
+    }
 };
 
-// d_Q partials: one wave per Q, the density tile of pair t held in registers.
-// grid = (ntp, ceil((q1-q0)/qblk)), 256 threads.
+// d_Q partials: one wave per Q, the density tile of pair t held in registers (lane l holds
+// elements 8l..8l+7 and 512+8l..512+8l+7).  Each wave takes two Q per pass, so two tiles
+// are in flight.  grid = (ntp, ceil((q1-q0)/qblk)), 256 threads.
 template <class BS>
 __global__ __launch_bounds__(256) void k_j1(BS B, const double* __restrict__ W, int ntp, int naux, int q0, int q1,
                                             int qblk, double* __restrict__ dpart) {
@@ -67,17 +99,34 @@ __global__ __launch_bounds__(256) void k_j1(BS B, const double* __restrict__ W, 
     const double* Wt = W + static_cast<size_t>(t) * TILE2;
     double w[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) w[k] = Wt[lane + 64 * k];
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w[8 * c + e] = Wt[512 * c + 8 * lane + e];
     const int qa = q0 + blockIdx.y * qblk;
     const int qb = min(q1, qa + qblk);
-    for (int Q = qa + wave; Q < qb; Q += 4) {
-        const size_t base = (static_cast<size_t>(Q) * ntp + t) * TILE2;
+    auto dot = [&](int Q) {
+        const size_t base = (static_cast<size_t>(Q) * ntp + t) * TILE2 + 8 * lane;
+        double v[16];
+        B.load8(base, v);
+        B.load8(base + 512, v + 8);
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) s = fma(B(base + lane + 64 * k), w[k], s);
+        for (int k = 0; k < 16; ++k) s = fma(v[k], w[k], s);
+        return s;
+    };
+    for (int Q = qa + wave; Q < qb; Q += 8) {
+        const int Q2 = Q + 4;
+        double s = dot(Q);
+        double s2 = Q2 < qb ? dot(Q2) : 0.0;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) dpart[static_cast<size_t>(t) * naux + Q] = s;
+        for (int off = 32; off > 0; off >>= 1) {
+            s += __shfl_xor(s, off);
+            s2 += __shfl_xor(s2, off);
+        }
+        if (lane == 0) {
+            dpart[static_cast<size_t>(t) * naux + Q] = s;
+            if (Q2 < qb) dpart[static_cast<size_t>(t) * naux + Q2] = s2;
+        }
     }
 }
 
@@ -89,28 +138,35 @@ __global__ void k_j1_reduce(const double* __restrict__ dpart, int ntp, int naux,
     d[Q] = s;
 }
 
-// J partials over a chunk of Q: grid = (ntp, nchunk), 256 threads, 4 elements per thread.
+// J partials over a chunk of Q: grid = (ntp, nchunk), 256 threads, 4 consecutive elements per
+// thread (vector loads), two Q per pass.
 template <class BS>
 __global__ __launch_bounds__(256) void k_j2(BS B, const double* __restrict__ d, int ntp, int naux, int qchunk,
                                             double* __restrict__ Jpart) {
     const int t = blockIdx.x, c = blockIdx.y;
     const int qa = c * qchunk, qb = min(naux, qa + qchunk);
-    const int e0 = threadIdx.x;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll 2
-    for (int Q = qa; Q < qb; ++Q) {
-        const double dq = d[Q];
+    const int e0 = 4 * threadIdx.x;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    int Q = qa;
+    for (; Q + 1 < qb; Q += 2) {
+        const double d0 = d[Q], d1 = d[Q + 1];
         const size_t base = (static_cast<size_t>(Q) * ntp + t) * TILE2 + e0;
-        a0 = fma(B(base), dq, a0);
-        a1 = fma(B(base + 256), dq, a1);
-        a2 = fma(B(base + 512), dq, a2);
-        a3 = fma(B(base + 768), dq, a3);
+        double v0[4], v1[4];
+        B.load4(base, v0);
+        B.load4(base + static_cast<size_t>(ntp) * TILE2, v1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] = fma(v1[k], d1, fma(v0[k], d0, a[k]));
+    }
+    if (Q < qb) {
+        const double d0 = d[Q];
+        double v0[4];
+        B.load4((static_cast<size_t>(Q) * ntp + t) * TILE2 + e0, v0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] = fma(v0[k], d0, a[k]);
     }
     double* o = Jpart + (static_cast<size_t>(c) * ntp + t) * TILE2 + e0;
-    o[0] = a0;
-    o[256] = a1;
-    o[512] = a2;
-    o[768] = a3;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = a[k];
 }
 
 __global__ void k_j_unpack(const double* __restrict__ Jpart, int nchunk, int ntp, const int* __restrict__ tidx,
