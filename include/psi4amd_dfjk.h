@@ -50,8 +50,8 @@ enum dfjk_status {
 
 enum dfjk_precision {
     DFJK_PREC_FP64 = 0,    /* FP64 MFMA everywhere */
-    DFJK_PREC_FP32 = 1,    /* K: FP32 MFMA, FP32 accumulate; J: FP64 */
-    DFJK_PREC_BF16X3 = 2   /* K: 3-term BF16 split products, FP32 accumulate; J: FP64 */
+    DFJK_PREC_FP32 = 1,    /* K: FP32 MFMA, FP32 accumulate; J: FP32 copy of B, FP64 accumulate */
+    DFJK_PREC_BF16X3 = 2   /* K: 3-term BF16 split products, FP32 accumulate; J: FP32 copy of B, FP64 accumulate */
 };
 
 enum dfjk_fitting {

@@ -210,6 +210,7 @@ struct dfjk_handle {
     // screening + tiles, metric (P|Q), metric factor, (P|mn), fit, attenuated tensor
     double setup_ms[6] = {0, 0, 0, 0, 0, 0};
     bool dM_ready = false;  // ws_G holds the metric factor M (left by the device factorisation)
+    bool metric_on_device = false;  // the last initialize computed (P|Q) on the device
     bool bw_comb = false;
     double bw_alpha = 1.0, bw_beta = 0.0;
     std::vector<double> fit_M;      // host metric factor, kept for rebuilding Bw after an omega change
@@ -414,7 +415,8 @@ struct Lap {
 // MFMA GEMMs); then X = L^-1 by block forward substitution, X_i = -L_ii^-1 (L_i,<i X_<i), with
 // the host inverses of the diagonal blocks.  M stays in ws_G for the fit (dM_ready); it is
 // returned on the host (row-major, lower) only when want_host.
-std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<double>& Jm, bool want_host) {
+std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<double>& Jm, bool want_host,
+                                         const double* dJm = nullptr) {
     const int n = h->naux;
     constexpr int NB = 256;
     const hipStream_t s = h->stream;
@@ -428,7 +430,10 @@ std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<doubl
     double* T = dT.get<double>(static_cast<size_t>(n) * NB);
     double* Li = dL.get<double>(static_cast<size_t>(NB) * NB);
     const size_t ld = static_cast<size_t>(n) * sizeof(double);
-    hcheck(hipMemcpyAsync(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload metric");
+    if (dJm)
+        hcheck(hipMemcpyAsync(A, dJm, nn * sizeof(double), hipMemcpyDeviceToDevice, s), "copy metric");
+    else
+        hcheck(hipMemcpyAsync(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload metric");
     hcheck(hipMemsetAsync(X, 0, nn * sizeof(double), s), "memset");
     std::vector<double> blk(NB * NB), invT(NB * NB);
     std::vector<std::vector<double>> dinv;  // host inverses of the diagonal blocks
@@ -486,7 +491,8 @@ std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<doubl
 // |a_i| < condition * max(|a_min|, |a_max|) and non-finite powers give 0, then
 // M = V diag(f) V^T, symmetric and full (the fit runs over the whole k range).  M is left in
 // ws_G (dM_ready) and returned on the host only when want_host.
-std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double>& Jm, bool want_host) {
+std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double>& Jm, bool want_host,
+                                        const double* dJm = nullptr) {
     const int n = h->naux;
     const size_t nn = static_cast<size_t>(n) * n;
     const hipStream_t s = h->stream;
@@ -496,7 +502,10 @@ std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double
     double* w = dw.get<double>(n);
     double* e = de.get<double>(n);
     rocblas_int* info = dinfo.get<rocblas_int>(1);
-    hcheck(hipMemcpyAsync(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload metric");
+    if (dJm)
+        hcheck(hipMemcpyAsync(A, dJm, nn * sizeof(double), hipMemcpyDeviceToDevice, s), "copy metric");
+    else
+        hcheck(hipMemcpyAsync(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload metric");
     rocblas_handle rb = nullptr;
     if (rocblas_create_handle(&rb) != rocblas_status_success) fail(DFJK_ERR_DEVICE, "rocblas_create_handle failed");
     struct RbGuard {
@@ -550,27 +559,40 @@ std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double
 
 // The metric factor M (L^-1, or psi4's truncated J^-1/2); on the host only when want_host
 // (else it is left in ws_G).
+bool gpu_metric(dfjk_handle* h, DevBuf& dJ);
+
 std::vector<double> metric_factor(dfjk_handle* h, bool want_host) {
     const int na = h->naux;
+    const size_t nn = static_cast<size_t>(na) * na;
     Lap lap;
-    std::vector<double> Jm(static_cast<size_t>(na) * na);
-    metric(h->aux, Jm.data(), h->nthreads);
+    // (P|Q) on the device when the kernels cover the auxiliary shells (then it stays there for
+    // the factorisation), else on the host
+    const bool host_factor = h->fitting == DFJK_FIT_CHOLESKY && std::getenv("PSI4AMD_HOST_FACTOR");
+    DevBuf dJ;
+    std::vector<double> Jm;
+    const bool on_dev = !host_factor && gpu_metric(h, dJ);
+    if (!on_dev) {
+        Jm.resize(nn);
+        metric(h->aux, Jm.data(), h->nthreads);
+    }
+    h->metric_on_device = on_dev;
     h->setup_ms[1] = lap();
     std::vector<double> M;
     h->fit_lower = h->fitting == DFJK_FIT_CHOLESKY;
     h->fit_kept = na;
+    const double* dJm = on_dev ? dJ.as<double>() : nullptr;
     if (h->fitting == DFJK_FIT_CHOLESKY) {
-        if (std::getenv("PSI4AMD_HOST_FACTOR")) {
+        if (host_factor) {
             h->dM_ready = false;
-            M.resize(static_cast<size_t>(na) * na);
+            M.resize(nn);
             const int bad = cholesky_lower(Jm.data(), na, h->nthreads);
             if (bad >= 0) cholesky_failed(bad);
             lower_inverse(Jm.data(), na, M.data(), h->nthreads);
         } else {
-            M = device_metric_factor(h, Jm, want_host);
+            M = device_metric_factor(h, Jm, want_host, dJm);
         }
     } else {
-        M = device_metric_eigen(h, Jm, want_host);
+        M = device_metric_eigen(h, Jm, want_host, dJm);
     }
     h->setup_ms[2] = lap();
     return M;
@@ -621,24 +643,12 @@ void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vect
 // then uses the host engine.
 std::vector<int64_t> tile_dst(const dfjk_handle* h);
 
-bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
-    if (std::getenv("PSI4AMD_HOST_INTS")) return false;
-    if (h->ooc) return false;  // a device-resident (P|mn) would not respect the memory budget
-    const Basis& pb = h->prim;
-    const Basis& ab = h->aux;
-    for (const Shell& sh : pb.shells)
-        if (sh.l > 3) return false;
-    for (const Shell& sh : ab.shells)
-        if (sh.l > 4) return false;
-    const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
-    const size_t abytes = static_cast<size_t>(h->naux) * ncolB * sizeof(double);
-    size_t free_b = 0, total_b = 0;
-    hcheck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    // the fit writes B (omega = 0) or the attenuated tensor: count it unless already allocated
-    const DevBuf& fit_dst = omega > 0.0 ? h->dBw : h->dB;
-    const size_t dst_bytes = fit_dst.bytes >= abytes ? 0 : abytes;
-    if (abytes + dst_bytes + (size_t(2) << 30) > free_b) return false;
-
+// Upload the shell tables of a "primary" basis pb and the auxiliary basis ab and run every
+// (la, lb | lc) class of the shell pairs pcls[la * 4 + lb] (la >= lb) against every auxiliary
+// function: out[P * p_stride + position of (m, n)] = (P|mn), positions from dst ([nfun(pb)^2],
+// or null: column m).  Waits for the stream.
+void run_three_index(dfjk_handle* h, const Basis& pb, const Basis& ab, const std::vector<std::vector<int2>>& pcls,
+                     const std::vector<int64_t>* dst, int64_t p_stride, double* out, double omega) {
     auto shells = [](const Basis& b, std::vector<GpuShell>& sh, std::vector<double>& e, std::vector<double>& c) {
         for (size_t i = 0; i < b.shells.size(); ++i) {
             const Shell& x = b.shells[i];
@@ -657,16 +667,6 @@ bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
     std::vector<double> pe, pc, ae, ac;
     shells(pb, ps, pe, pc);
     shells(ab, as, ae, ac);
-    // significant shell pairs (the host engine's M >= N set), higher l first, by class
-    const int np = static_cast<int>(pb.shells.size());
-    std::vector<std::vector<int2>> pcls(16);
-    for (int M = 0; M < np; ++M)
-        for (int N = 0; N <= M; ++N) {
-            if (!h->shell_mask[static_cast<size_t>(M) * np + N]) continue;
-            int X = M, Y = N;
-            if (pb.shells[X].l < pb.shells[Y].l) std::swap(X, Y);
-            pcls[pb.shells[X].l * 4 + pb.shells[Y].l].push_back(make_int2(X, Y));
-        }
     std::vector<std::vector<int2>> acls(5);
     for (size_t i = 0; i < ab.shells.size(); ++i)
         for (int k = 0; k < ab.shells[i].nfunc(); ++k) acls[ab.shells[i].l].push_back(make_int2(static_cast<int>(i), k));
@@ -693,7 +693,6 @@ bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
     int bnt = 0, bncol = 0;
     double bh = 0.0, btmax = 0.0;
     const double* btab = boys_grid(&bnt, &bncol, &bh, &btmax);
-    const std::vector<int64_t> dst = tile_dst(h);
 
     DevBuf d_ps, d_as, d_pe, d_pc, d_ae, d_ac, d_c2s, d_boys, d_dst, d_pairs, d_afun;
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
@@ -708,12 +707,10 @@ bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
     up(d_ac, ac.data(), ac.size() * sizeof(double));
     up(d_c2s, c2s.data(), c2s.size() * sizeof(double));
     up(d_boys, btab, static_cast<size_t>(bnt) * bncol * sizeof(double));
-    up(d_dst, dst.data(), dst.size() * sizeof(int64_t));
+    if (dst) up(d_dst, dst->data(), dst->size() * sizeof(int64_t));
     up(d_pairs, pall.data(), std::max<size_t>(1, pall.size()) * sizeof(int2));
     up(d_afun, aall.data(), std::max<size_t>(1, aall.size()) * sizeof(int2));
 
-    double* A = Abuf.get<double>(static_cast<size_t>(h->naux) * ncolB);
-    hcheck(hipMemsetAsync(A, 0, abytes, h->stream), "memset A");
     a.prim = d_ps.as<GpuShell>();
     a.aux = d_as.as<GpuShell>();
     a.pexp = d_pe.as<double>();
@@ -728,10 +725,10 @@ bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
     a.boys.h = bh;
     a.boys.inv_h = 1.0 / bh;
     a.boys.tmax = btmax;
-    a.dst = d_dst.as<int64_t>();
-    a.p_stride = static_cast<int64_t>(ncolB);
-    a.nbf = h->nbf;
-    a.out = A;
+    a.dst = dst ? d_dst.as<int64_t>() : nullptr;
+    a.p_stride = p_stride;
+    a.nbf = pb.nbf;
+    a.out = out;
     a.omega = omega;
     const int2* dp = d_pairs.as<int2>();
     const int2* da = d_afun.as<int2>();
@@ -747,6 +744,68 @@ bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
             }
         }
     hcheck(hipStreamSynchronize(h->stream), "three-index integrals");
+}
+
+bool gpu_three_index(dfjk_handle* h, double omega, DevBuf& Abuf) {
+    if (std::getenv("PSI4AMD_HOST_INTS")) return false;
+    if (h->ooc) return false;  // a device-resident (P|mn) would not respect the memory budget
+    const Basis& pb = h->prim;
+    const Basis& ab = h->aux;
+    for (const Shell& sh : pb.shells)
+        if (sh.l > 3) return false;
+    for (const Shell& sh : ab.shells)
+        if (sh.l > 4) return false;
+    const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
+    const size_t abytes = static_cast<size_t>(h->naux) * ncolB * sizeof(double);
+    size_t free_b = 0, total_b = 0;
+    hcheck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    // the fit writes B (omega = 0) or the attenuated tensor: count it unless already allocated
+    const DevBuf& fit_dst = omega > 0.0 ? h->dBw : h->dB;
+    const size_t dst_bytes = fit_dst.bytes >= abytes ? 0 : abytes;
+    if (abytes + dst_bytes + (size_t(2) << 30) > free_b) return false;
+    // significant shell pairs (the host engine's M >= N set), higher l first, by class
+    const int np = static_cast<int>(pb.shells.size());
+    std::vector<std::vector<int2>> pcls(16);
+    for (int M = 0; M < np; ++M)
+        for (int N = 0; N <= M; ++N) {
+            if (!h->shell_mask[static_cast<size_t>(M) * np + N]) continue;
+            int X = M, Y = N;
+            if (pb.shells[X].l < pb.shells[Y].l) std::swap(X, Y);
+            pcls[pb.shells[X].l * 4 + pb.shells[Y].l].push_back(make_int2(X, Y));
+        }
+    const std::vector<int64_t> dst = tile_dst(h);
+    double* A = Abuf.get<double>(static_cast<size_t>(h->naux) * ncolB);
+    hcheck(hipMemsetAsync(A, 0, abytes, h->stream), "memset A");
+    run_three_index(h, pb, ab, pcls, &dst, static_cast<int64_t>(ncolB), A, omega);
+    return true;
+}
+
+// The Coulomb metric (P|Q) on the device (fittingmetric.cc:115-155, the Libint2 two-centre
+// step), by the three-centre kernels with the pair (Q, unit s function): (P|Q 1) = (P|Q).
+// dJ [naux][naux], both triangles.  Returns false (nothing launched) for auxiliary l > 3 (the
+// pair slot of the kernels), with PSI4AMD_HOST_INTS or PSI4AMD_HOST_METRIC set; the caller then
+// uses the host engine.
+bool gpu_metric(dfjk_handle* h, DevBuf& dJ) {
+    if (std::getenv("PSI4AMD_HOST_INTS") || std::getenv("PSI4AMD_HOST_METRIC")) return false;
+    const Basis& ab = h->aux;
+    for (const Shell& sh : ab.shells)
+        if (sh.l > 3) return false;
+    // the auxiliary shells at their own function offsets, plus one unit s shell (exponent 0,
+    // coefficient 1: its product with any Gaussian is that Gaussian)
+    Basis pb = ab;
+    Shell unit;
+    unit.l = 0;
+    unit.pure = ab.shells.empty() ? true : ab.shells[0].pure;
+    unit.exps = {0.0};
+    unit.coefs = {1.0};
+    pb.shells.push_back(unit);
+    pb.finalize();
+    const int iu = static_cast<int>(pb.shells.size()) - 1;
+    std::vector<std::vector<int2>> pcls(16);
+    for (int Q = 0; Q < iu; ++Q) pcls[pb.shells[Q].l * 4].push_back(make_int2(Q, iu));
+    const size_t nn = static_cast<size_t>(h->naux) * h->naux;
+    double* J = dJ.get<double>(nn);
+    run_three_index(h, pb, ab, pcls, nullptr, h->naux, J, 0.0);
     return true;
 }
 
@@ -1192,15 +1251,17 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     // resident lower-precision copies (in core); the fixed-workspace entry point allocates
     // none, so an earlier allocating compute at this precision must have made them
     if (!h->ooc) {
-        const bool miss = (prec == DFJK_PREC_FP32 && !h->dB32.p) ||
-                          (prec == DFJK_PREC_BF16X3 && (!h->dBh.p || (doK && !h->dBhT.p)));
+        const bool miss = (prec != DFJK_PREC_FP64 && doJ && !h->dB32.p) ||
+                          (prec == DFJK_PREC_BF16X3 && doK && (!h->dBh.p || !h->dBhT.p));
         if (wk.fixed() && miss)
             fail(DFJK_ERR_STATE, "dfjk_compute_device_noalloc: the lower-precision copy of B for this precision does "
                                  "not exist yet; run one dfjk_compute_device at this precision first");
-        if (prec == DFJK_PREC_FP32) ensure_B32(h);
-        if (prec == DFJK_PREC_BF16X3) {
+        // J reads the FP32 copy in both mixed modes (2^-24 relative, the same bytes as the BF16
+        // pair's 2^-17): the J error of a density change would otherwise dominate its K error
+        if (prec == DFJK_PREC_FP32 || (prec == DFJK_PREC_BF16X3 && doJ)) ensure_B32(h);
+        if (prec == DFJK_PREC_BF16X3 && doK) {
             ensure_Bbf(h);
-            if (doK) ensure_BbfT(h);
+            ensure_BbfT(h);
         }
     }
 
@@ -1247,7 +1308,8 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         }
         if (h->ooc) {
             // two FP64 stage buffers (plus their converted copies) share the budget with X
-            const size_t pq = qs * (sizeof(double) + (prec == DFJK_PREC_FP64 ? 0 : 2 * sizeof(float))) * 2;
+            const size_t conv = prec == DFJK_PREC_FP32 ? 4 : prec == DFJK_PREC_BF16X3 ? 12 : 0;  // FP32, BF16 pairs
+            const size_t pq = qs * (2 * sizeof(double) + conv);
             const size_t stage_budget = std::max<size_t>(pq, std::min<size_t>(size_t(8) << 30, budget / 2));
             qblk = std::min<int>(qblk, static_cast<int>(std::max<size_t>(1, stage_budget / pq)));
         }
@@ -1342,13 +1404,9 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     // tensor, or its lower-precision copy); out of core it is accumulated block by block below.
     if (doJ && !h->ooc) {
         JSource js;
-        if (prec == DFJK_PREC_FP32) {
+        if (prec != DFJK_PREC_FP64) {
             js.kind = JSource::F32;
             js.a = h->dB32.p;
-        } else if (prec == DFJK_PREC_BF16X3) {
-            js.kind = JSource::BF2;
-            js.a = h->dBh.p;
-            js.b = h->dBl.p;
         } else {
             js.a = h->dB.p;
         }
@@ -1387,14 +1445,12 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         if (!h->ooc) {
             const size_t off = static_cast<size_t>(qa) * qs;
             b64 = h->dB.as<double>() + off;
-            if (prec == DFJK_PREC_FP32) b32 = h->dB32.as<float>() + off;
-            if (prec == DFJK_PREC_BF16X3) {
+            if (prec != DFJK_PREC_FP64 && doJ) b32 = h->dB32.as<float>() + off;
+            if (prec == DFJK_PREC_BF16X3 && doK) {
                 bh = h->dBh.as<uint16_t>() + off;
                 bl = h->dBl.as<uint16_t>() + off;
-                if (doK) {
-                    bhT = h->dBhT.as<uint16_t>() + off;
-                    blT = h->dBlT.as<uint16_t>() + off;
-                }
+                bhT = h->dBhT.as<uint16_t>() + off;
+                blT = h->dBlT.as<uint16_t>() + off;
             }
         } else {
             if (blk + 1 < nblocks) stage_copy(blk + 1);
@@ -1402,34 +1458,29 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             double* st = h->ws_stage[blk & 1].as<double>();
             b64 = st;
             hipEvent_t e0 = tm.begin();
-            if (prec == DFJK_PREC_FP32) {
+            if (prec == DFJK_PREC_FP32 || (prec == DFJK_PREC_BF16X3 && doJ)) {
                 float* c = wk.get<float>(h->ws_stage32, qs * qblk);
                 launch_to_f32(st, c, nel, s);
                 b32 = c;
-            } else if (prec == DFJK_PREC_BF16X3) {
+            }
+            if (prec == DFJK_PREC_BF16X3 && doK) {
                 uint16_t* c = wk.get<uint16_t>(h->ws_stagebf, 4 * qs * qblk);
                 const size_t cap = qs * qblk;
                 launch_to_bf_split(st, c, c + cap, nel, s);
+                launch_to_bf_split_t(st, c + 2 * cap, c + 3 * cap, nel, s);
                 bh = c;
                 bl = c + cap;
-                if (doK) {
-                    launch_to_bf_split_t(st, c + 2 * cap, c + 3 * cap, nel, s);
-                    bhT = c + 2 * cap;
-                    blT = c + 3 * cap;
-                }
+                bhT = c + 2 * cap;
+                blT = c + 3 * cap;
             }
             tm.end(T_OTHER, e0);
         }
 
         if (doJ && h->ooc) {
             JSource js;
-            if (prec == DFJK_PREC_FP32) {
+            if (prec != DFJK_PREC_FP64) {
                 js.kind = JSource::F32;
                 js.a = b32;
-            } else if (prec == DFJK_PREC_BF16X3) {
-                js.kind = JSource::BF2;
-                js.a = bh;
-                js.b = bl;
             } else {
                 js.a = b64;
             }

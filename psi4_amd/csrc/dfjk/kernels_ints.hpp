@@ -30,7 +30,9 @@ struct ThreeIndexArgs {
     int c2s_off[5] = {0, 0, 0, 0, 0};
     int prim_pure = 1, aux_pure = 1;
     BoysGrid boys;
-    const int64_t* dst = nullptr;     // [nbf*nbf] tiled-layout position of (m,n) or -1
+    const int64_t* dst = nullptr;     // [nbf*nbf] tiled-layout position of (m,n) or -1; null: the value
+                                      // of (m, n) goes to column m (pairs of a shell with a unit s
+                                      // shell, as for the two-centre metric (P|Q))
     int64_t p_stride = 0;             // elements per auxiliary function in out
     int nbf = 0;
     double* out = nullptr;            // [naux][p_stride]

@@ -319,6 +319,10 @@ __global__ __launch_bounds__(64) void k_three_index(ThreeIndexArgs a, const int2
                 }
             }
             const int m = sa.foff + ma, n = sb.foff + mb;
+            if (!a.dst) {
+                a.out[row + m] = v;
+                continue;
+            }
             const int64_t d1 = a.dst[static_cast<int64_t>(m) * a.nbf + n];
             const int64_t d2 = a.dst[static_cast<int64_t>(n) * a.nbf + m];
             if (d1 >= 0) a.out[row + d1] = v;

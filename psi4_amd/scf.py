@@ -17,6 +17,15 @@ stays symmetric.  A full FP64 rebuild runs
 every ``incfock_full_every`` iterations (psi4's INCFOCK_FULL_FOCK_EVERY
 pattern, libfock/DirectJK.cc:333-349), and once the orbital gradient falls
 below ``mixed_until`` every remaining iteration is a full FP64 build.
+
+Per-build error bound: the energy error of a low-precision increment grows with
+the size of the density change, |dE| <= c_prec ||dD||_F (c_prec measured on
+water-cluster SCF trajectories, MIXED_ERROR_PER_DD), and the increments since the
+last FP64 build add up in J and K.  An increment is therefore taken in low
+precision only while the running sum of these bounds stays within ``build_tol``
+(1e-6 Eh); otherwise the iteration is a full FP64 build, which resets the sum.
+Every build's J and K are thus within ``build_tol`` of the FP64 build of the
+same orbitals.
 """
 from __future__ import annotations
 
@@ -30,6 +39,14 @@ from . import ints
 from .core import JK, PREC_BF16X3, PREC_FP32, PREC_FP64
 
 _PREC = {None: PREC_FP64, "fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+# Bound on |E_build(mixed) - E_build(FP64)| per unit ||dD||_F of one cross-form increment
+# J(dD), K(dD) (Eh).  Measured along the plain FP64 SCF trajectory of the seeded (H2O)_48
+# cluster (cc-pVDZ/cc-pVDZ-jkfit, 23 increments, ||dD||_F from 100 down to 1e-5): the ratio
+# never exceeded 4.1e-6 for BF16x3 (K dominated; J reads the FP32 copy of B) and 6.1e-7 for
+# FP32; the synthetic nbf=1500 increment gives 8e-6 for BF16x3.  The bounds below keep a
+# margin above all of them.
+MIXED_ERROR_PER_DD = {"fp32": 2e-6, "bf16x3": 1e-5}
 
 
 class SCFConvergenceError(RuntimeError):
@@ -121,7 +138,7 @@ class RHF:
 
     def __init__(self, mol, basis="cc-pvdz", df_basis_scf=None, jk=None, e_convergence=1e-10, d_convergence=1e-8,
                  maxiter=100, diis_max_vecs=10, mixed_precision=None, mixed_until=1e-7, incfock_full_every=8,
-                 fail_on_maxiter=True, jk_callback=None, perturb_dipole=None, s_tolerance=1e-7):
+                 fail_on_maxiter=True, jk_callback=None, perturb_dipole=None, s_tolerance=1e-7, build_tol=1e-6):
         from .basisset import BasisSet
         self.mol = mol
         self.basis = BasisSet.build(mol, basis)
@@ -135,6 +152,7 @@ class RHF:
         if mixed_precision not in _PREC:
             raise ValueError(f"unknown precision {mixed_precision}")
         self.mixed_until = mixed_until
+        self.build_tol = float(build_tol)
         self.full_every = max(1, int(incfock_full_every))
         self.fail_on_maxiter = fail_on_maxiter
         self.jk_callback = jk_callback  # called as cb(iteration, C_occ, J, K, kind) after every build
@@ -254,24 +272,32 @@ class RHF:
         E_old = 0.0
         history = []
         J = K = None
-        C_prev = None
+        C_prev = D_prev = None
         since_full = 0
+        err_bound = 0.0          # sum of the increments' error bounds since the last FP64 build
         mixed_on = self.mixed is not None
         t_jk = 0.0
         for it in range(1, self.maxiter + 1):
             Cocc = C[:, :self.nocc].contiguous()
             D = Cocc @ Cocc.T
             tj = time.perf_counter()
-            if J is None or not mixed_on or since_full + 1 >= self.full_every:
+            bound = None
+            if J is not None and mixed_on and since_full + 1 < self.full_every:
+                bound = MIXED_ERROR_PER_DD[self.mixed] * float(torch.linalg.norm(D - D_prev))
+                if err_bound + bound > self.build_tol:
+                    bound = None         # this increment could exceed the per-build bound
+            if bound is None:
                 J, K = self._full_build(Cocc)
                 kind = "fp64"
                 since_full = 0
+                err_bound = 0.0
             else:
                 dJ, dK = self._delta_build(Cocc, C_prev, S @ Cocc)
                 J = J + dJ
                 K = K + dK
                 kind = self.mixed
                 since_full += 1
+                err_bound += bound
             dt_jk = time.perf_counter() - tj
             t_jk += dt_jk
             if self.jk_callback is not None:
@@ -281,7 +307,7 @@ class RHF:
             FDS = F @ D @ S
             grad = X.T @ (FDS - FDS.T) @ X
             d_rms = float(torch.sqrt(torch.mean(grad * grad)))
-            history.append(dict(iteration=it, energy=E, d_rms=d_rms, build=kind, jk_s=dt_jk))
+            history.append(dict(iteration=it, energy=E, d_rms=d_rms, build=kind, jk_s=dt_jk, err_bound=err_bound))
             converged = abs(E - E_old) < self.e_conv and d_rms < self.d_conv
             if converged and kind == "fp64":
                 return self._result(E, C, eps, D, J, K, F, it, history, t_setup, t_jk, t0)
@@ -289,7 +315,7 @@ class RHF:
                 mixed_on = False         # FP64 finishing iterations from here on
             diis.add(F, grad)
             eps, C = diag(diis.extrapolate())
-            C_prev = Cocc
+            C_prev, D_prev = Cocc, D
             E_old = E
         if self.fail_on_maxiter:
             raise SCFConvergenceError(f"SCF did not converge in {self.maxiter} iterations")

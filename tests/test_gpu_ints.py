@@ -4,8 +4,13 @@ The metric factor L^-1 (J = L L^T) from the device's blocked Cholesky and block 
 substitution gives the same fitted tensor as the host factorisation ((H2O)_16, naux 1856 in
 256-row blocks with a partial last block; 1e-11 of the largest element).
 
+The metric (P|Q) computed on the device gives the same fitted tensor as the host engine's
+metric ((H2O)_16; 1e-11 of the largest element: the fit amplifies last-bit differences of
+the metric by its condition number, about 1e7 here).
+
 The fitted tensor B = L^-1 (P|mn) built from the device integrals equals the one built from
-the host engine's integrals (same device fit) to 1e-12 of the largest element, for water and
+the host engine's integrals (same host metric, same device fit) to 1e-12 of the largest
+element, for water and
 (H2O)_8 in cc-pVDZ / cc-pVDZ-jkfit (s..d | s..f) and water in cc-pVTZ / cc-pVTZ-jkfit
 (s..f | s..g), and likewise the attenuated tensor of wK (omega = 0.3).  The host engine's
 integrals are themselves pinned by psi4's energies and gradients.
@@ -21,15 +26,24 @@ from conftest import announce
 
 pytestmark = pytest.mark.gpu
 
+# host integrals imply the host metric: the three-index comparisons hold the metric on the
+# host in both runs, so the two tensors differ only by their (P|mn)
+HOST_METRIC = ("PSI4AMD_HOST_METRIC",)
 
-def _tensor(mol, basis, device, omega=None, key="PSI4AMD_HOST_INTS", info=None):
+
+def _tensor(mol, basis, device, omega=None, key="PSI4AMD_HOST_INTS", info=None, fixed=()):
+    """B (and the attenuated tensor) with the step named by key on the device or, with
+    device False, on the host; the switches in fixed are set in both runs."""
     from psi4_amd.basisset import BasisSet
     from psi4_amd.core import JK
     prim = BasisSet.build(mol, basis)
     aux = BasisSet.build(mol, basis + "-jkfit")
-    old = os.environ.pop(key, None)
+    keys = (key,) + tuple(fixed)
+    saved = {k: os.environ.pop(k, None) for k in keys}
     if not device:
         os.environ[key] = "1"
+    for k in fixed:
+        os.environ[k] = "1"
     try:
         jk = JK.build(prim, aux, do_wK=bool(omega), memory=8_000_000_000)
         jk.set_print(0)
@@ -44,9 +58,10 @@ def _tensor(mol, basis, device, omega=None, key="PSI4AMD_HOST_INTS", info=None):
         B = jk.get_B_dense(0, naux)
         Bw = jk.get_Bw_dense(0, naux) if omega else None
     finally:
-        os.environ.pop(key, None)
-        if old is not None:
-            os.environ[key] = old
+        for k, v in saved.items():
+            os.environ.pop(k, None)
+            if v is not None:
+                os.environ[k] = v
     return B, Bw, dt
 
 
@@ -61,9 +76,9 @@ def test_device_three_index_matches_host(gpu_available, case):
     else:
         mol, basis = M.water_zmat(0.96, 104.5), ("cc-pvtz" if case == "water-tz" else "cc-pvdz")
     announce(f"device (P|mn) {case}: device integrals")
-    Bd, _, td = _tensor(mol, basis, True)
+    Bd, _, td = _tensor(mol, basis, True, fixed=HOST_METRIC)
     announce(f"device (P|mn) {case}: host integrals")
-    Bh, _, th = _tensor(mol, basis, False)
+    Bh, _, th = _tensor(mol, basis, False, fixed=HOST_METRIC)
     e = _rel(Bd, Bh)
     print(f"  {case}: |B_dev - B_host|/max {e:.2e}; initialize {td:.2f} s device ints, {th:.2f} s host ints", flush=True)
     assert e < 1e-12
@@ -72,8 +87,8 @@ def test_device_three_index_matches_host(gpu_available, case):
 def test_device_three_index_attenuated(gpu_available):
     mol = M.water_cluster((2, 1, 1), seed=3)
     announce("device (P|w|mn), omega 0.3, (H2O)_2 cc-pVDZ")
-    Bd, Bwd, _ = _tensor(mol, "cc-pvdz", True, omega=0.3)
-    Bh, Bwh, _ = _tensor(mol, "cc-pvdz", False, omega=0.3)
+    Bd, Bwd, _ = _tensor(mol, "cc-pvdz", True, omega=0.3, fixed=HOST_METRIC)
+    Bh, Bwh, _ = _tensor(mol, "cc-pvdz", False, omega=0.3, fixed=HOST_METRIC)
     print(f"  B {_rel(Bd, Bh):.2e}  Bw {_rel(Bwd, Bwh):.2e}", flush=True)
     assert _rel(Bd, Bh) < 1e-12 and _rel(Bwd, Bwh) < 1e-12
 
@@ -82,8 +97,8 @@ def test_device_metric_factor_matches_host(gpu_available):
     mol = M.water_cluster((4, 2, 2), seed=4)
     announce("metric factor (H2O)_16 cc-pVDZ-jkfit: device blocked Cholesky vs host")
     info = {}
-    Bd, _, td = _tensor(mol, "cc-pvdz", True, key="PSI4AMD_HOST_FACTOR", info=info)
-    Bh, _, th = _tensor(mol, "cc-pvdz", False, key="PSI4AMD_HOST_FACTOR")
+    Bd, _, td = _tensor(mol, "cc-pvdz", True, key="PSI4AMD_HOST_FACTOR", info=info, fixed=HOST_METRIC)
+    Bh, _, th = _tensor(mol, "cc-pvdz", False, key="PSI4AMD_HOST_FACTOR", fixed=HOST_METRIC)
     e = _rel(Bd, Bh)
     t = info["setup_ms"]
     print(f"  naux {info['naux']}: |B_dev - B_host|/max {e:.2e}; initialize {td:.2f} s device factor, "
@@ -91,3 +106,18 @@ def test_device_metric_factor_matches_host(gpu_available):
     assert info["naux"] == 1856
     assert e < 1e-11
     assert t["metric"] > 0 and t["metric_factor"] > 0 and t["fit"] > 0 and t["attenuated"] == 0
+
+
+def test_device_metric_matches_host(gpu_available):
+    """The two-centre metric (P|Q) on the device (the three-centre kernels with a unit s
+    partner) against the host engine's: the fitted tensor of (H2O)_16 (naux 1856, s..f
+    auxiliary shells) agrees to 1e-11 of its largest element."""
+    mol = M.water_cluster((4, 2, 2), seed=4)
+    announce("metric (P|Q) (H2O)_16 cc-pVDZ-jkfit: device vs host")
+    info_d, info_h = {}, {}
+    Bd, _, td = _tensor(mol, "cc-pvdz", True, key="PSI4AMD_HOST_METRIC", info=info_d)
+    Bh, _, th = _tensor(mol, "cc-pvdz", False, key="PSI4AMD_HOST_METRIC", info=info_h)
+    e = _rel(Bd, Bh)
+    print(f"  |B_dev_metric - B_host_metric|/max {e:.2e}; metric phase {info_d['setup_ms']['metric']:.1f} ms "
+          f"(device) vs {info_h['setup_ms']['metric']:.1f} ms (host)", flush=True)
+    assert e < 1e-11

@@ -2,9 +2,8 @@
 
 FP64 J and K must match the oracle to 1e-11 max-abs (the bar for plain FP64).
 Mixed-precision K is checked against the FP64 oracle with tolerances relative
-to max|K|: FP32 MFMA 2e-6, 3-term BF16 split 2e-4.  In the mixed modes J
-accumulates in FP64 from the FP32 copy of B (1e-7 of max|J|) or from the BF16
-hi + lo pair (2e-5 of max|J|).
+to max|K|: FP32 MFMA 2e-6, 3-term BF16 split 2e-4.  In both mixed modes J
+accumulates in FP64 from the FP32 copy of B (1e-7 of max|J|).
 """
 import numpy as np
 import pytest
@@ -16,8 +15,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-11
 REL_K = {1: 2e-6, 2: 2e-4}  # PREC_FP32, PREC_BF16X3
-# J accumulates in FP64 but reads the FP32 copy / the BF16 hi+lo pair of B in the mixed modes
-REL_J = {1: 1e-7, 2: 2e-5}
+# J accumulates in FP64 but reads the FP32 copy of B in the mixed modes
+REL_J = {1: 1e-7, 2: 1e-7}
 
 
 @pytest.fixture(scope="module")
