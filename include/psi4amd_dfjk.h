@@ -123,8 +123,16 @@ int dfjk_memory_estimate(dfjk_handle* h, size_t* doubles);
 int dfjk_device_footprint(dfjk_handle* h, size_t* doubles);
 /* The Schwarz function-pair mask of DFHelper::prepare_sparsity (dfhelper.cc:299-420):
  * mask[m*nbf + n] = 1 when (mn|mn) >= cutoff^2 / max (mn|mn), else 0 (nbf x nbf bytes), and the
- * cutoff it was built with.  Needs no device; either output may be NULL. */
+ * cutoff it was built with.  The (mn|mn) are computed on the device when one is present, by
+ * the host engine otherwise (so this also works without a GPU); either output may be NULL. */
 int dfjk_get_pair_mask(dfjk_handle* h, uint8_t* mask, double* cutoff);
+/* The screening integrals behind that mask, as psi4's TwoBodyAOInt keeps them
+ * (libmints/twobody.h:128-142, 232-238: function_pair_values_, shell_pair_value(), max_integral();
+ * filled by dfhelper.cc:329-356): fun_values[m*nbf + n] = |(mn|mn)| (nbf x nbf),
+ * shell_values[M*nshell + N] = its maximum over the shell pair (nshell x nshell), the overall
+ * maximum, and 1 if the device computed them.  Basis-built handles only; any output may be NULL. */
+int dfjk_get_schwarz(dfjk_handle* h, double* fun_values, double* shell_values, double* max_integral,
+                     int* on_device);
 /* The metric fit in use: enum dfjk_fitting, the condition, and the number of metric
  * eigenvalues kept (psi4's Matrix::power return value; naux for the Cholesky fit, 0 before
  * initialize).  Any output may be NULL. */

@@ -51,7 +51,8 @@ def build_ints(verbose=False):
 
 
 DFJK_SOURCES = ("kernels.hip", "kernels_k.hip", "kernels_ints.hip", "kernels_ints_l0.hip", "kernels_ints_l1.hip",
-                "kernels_ints_l2.hip", "kernels_ints_l3.hip", "kernels_ints_l4.hip", "dfjk.hip")
+                "kernels_ints_l2.hip", "kernels_ints_l3.hip", "kernels_ints_l4.hip", "kernels_schwarz.hip",
+                "kernels_schwarz_f.hip", "dfjk.hip")
 
 
 def build_dfjk(verbose=False):

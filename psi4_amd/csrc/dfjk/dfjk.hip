@@ -27,6 +27,7 @@
 #include "kernels.hpp"
 #include "kernels_ints.hpp"
 #include "kernels_k.hpp"
+#include "kernels_schwarz.hpp"
 #include "psi4amd_dfjk.h"
 
 // Workspace of dfjk_compute_device_noalloc: one array per GPU, held from the library's load
@@ -189,7 +190,7 @@ struct dfjk_handle {
     std::vector<int> otile_of_sym, otile_of_all;
     int ntile_sym = 0, ntile_all = 0;
     // resident tensor and its lower-precision copies
-    DevBuf dB, dB32, dBh, dBl, dBhT, dBlT;
+    DevBuf dB, dB32;
     DevBuf dZero;  // one zero TILE x TILE FP64 block, read in place of absent blocks
     // out of core (DFHelper AO_core false, dfhelper.cc:216-260): B lives in pinned host memory
     // and each compute streams it through two device stage buffers, Q-block by Q-block
@@ -198,7 +199,7 @@ struct dfjk_handle {
     double* hB = nullptr;               // pinned host B, b_doubles
     hipStream_t cstream = nullptr;      // copy stream (default priority, ordered by events)
     hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
-    DevBuf ws_stage[2], ws_stage32, ws_stagebf;
+    DevBuf ws_stage[2], ws_stage32;
     // range-separated exchange: Bw = M (P|erf(omega r)/r|mn), same layout and fit as B
     DevBuf dBw;
     double bw_omega = -1.0;         // omega dBw was built for (< 0: none)
@@ -211,6 +212,11 @@ struct dfjk_handle {
     double setup_ms[6] = {0, 0, 0, 0, 0, 0};
     bool dM_ready = false;  // ws_G holds the metric factor M (left by the device factorisation)
     bool metric_on_device = false;  // the last initialize computed (P|Q) on the device
+    // the Schwarz diagonal the masks were built from: |(mn|mn)| [nbf][nbf], its shell maxima
+    // [nshell][nshell] and overall maximum, and whether the device computed it
+    std::vector<double> schwarz_fun, schwarz_shell;
+    double schwarz_max = 0.0;
+    bool schwarz_on_device = false;
     bool bw_comb = false;
     double bw_alpha = 1.0, bw_beta = 0.0;
     std::vector<double> fit_M;      // host metric factor, kept for rebuilding Bw after an omega change
@@ -353,12 +359,34 @@ void upload_tiles(dfjk_handle* h) {
 
 size_t b_doubles(const dfjk_handle* h) { return static_cast<size_t>(h->naux) * h->ntp * TILE2; }
 
-// Schwarz screening exactly as dfhelper.cc:299-420: tolerance = cutoff^2 / max (mn|mn).
+bool gpu_schwarz(dfjk_handle* h, double* fun_max);
+
+// Schwarz screening exactly as dfhelper.cc:299-420: tolerance = cutoff^2 / max (mn|mn).  The
+// diagonal (mn|mn) comes from the device when one is present (the host engine otherwise); the
+// shell maxima, the global maximum and the masks are formed here from it.
 void prepare_sparsity(dfjk_handle* h) {
     const int n = h->nbf;
     const int ns = static_cast<int>(h->prim.shells.size());
-    std::vector<double> fun_max(static_cast<size_t>(n) * n), shell_max(static_cast<size_t>(ns) * ns);
-    const double max_val = schwarz_diagonal(h->prim, fun_max.data(), shell_max.data(), h->nthreads);
+    std::vector<double>& fun_max = h->schwarz_fun;
+    std::vector<double>& shell_max = h->schwarz_shell;
+    fun_max.assign(static_cast<size_t>(n) * n, 0.0);
+    shell_max.assign(static_cast<size_t>(ns) * ns, 0.0);
+    double max_val = 0.0;
+    h->schwarz_on_device = gpu_schwarz(h, fun_max.data());
+    if (h->schwarz_on_device) {
+        for (int M = 0; M < ns; ++M)
+            for (int N = 0; N < ns; ++N) {
+                double v = 0.0;
+                for (int m = h->prim.offset[M]; m < h->prim.offset[M] + h->prim.shells[M].nfunc(); ++m)
+                    for (int k = h->prim.offset[N]; k < h->prim.offset[N] + h->prim.shells[N].nfunc(); ++k)
+                        v = std::max(v, fun_max[static_cast<size_t>(m) * n + k]);
+                shell_max[static_cast<size_t>(M) * ns + N] = v;
+                max_val = std::max(max_val, v);
+            }
+    } else {
+        max_val = schwarz_diagonal(h->prim, fun_max.data(), shell_max.data(), h->nthreads);
+    }
+    h->schwarz_max = max_val;
     const double tol = h->cutoff * h->cutoff / max_val;
     h->fun_mask.assign(static_cast<size_t>(n) * n, 0);
     h->shell_mask.assign(static_cast<size_t>(ns) * ns, 0);
@@ -647,8 +675,12 @@ std::vector<int64_t> tile_dst(const dfjk_handle* h);
 // (la, lb | lc) class of the shell pairs pcls[la * 4 + lb] (la >= lb) against every auxiliary
 // function: out[P * p_stride + position of (m, n)] = (P|mn), positions from dst ([nfun(pb)^2],
 // or null: column m).  Waits for the stream.
-void run_three_index(dfjk_handle* h, const Basis& pb, const Basis& ab, const std::vector<std::vector<int2>>& pcls,
-                     const std::vector<int64_t>* dst, int64_t p_stride, double* out, double omega) {
+// Device copies of the shell tables of a "primary" basis pb (and an auxiliary basis ab), the
+// solid-harmonic rows l = 0..4 and the host engine's Boys grid; fills the table fields of a.
+struct IntTables {
+    DevBuf ps, as, pe, pc, ae, ac, c2s, boys;
+};
+void upload_int_tables(const Basis& pb, const Basis* ab, IntTables& t, ThreeIndexArgs& a) {
     auto shells = [](const Basis& b, std::vector<GpuShell>& sh, std::vector<double>& e, std::vector<double>& c) {
         for (size_t i = 0; i < b.shells.size(); ++i) {
             const Shell& x = b.shells[i];
@@ -666,7 +698,99 @@ void run_three_index(dfjk_handle* h, const Basis& pb, const Basis& ab, const std
     std::vector<GpuShell> ps, as;
     std::vector<double> pe, pc, ae, ac;
     shells(pb, ps, pe, pc);
-    shells(ab, as, ae, ac);
+    if (ab) shells(*ab, as, ae, ac);
+    std::vector<double> c2s;
+    for (int l = 0; l <= 4; ++l) {
+        a.c2s_off[l] = static_cast<int>(c2s.size());
+        const std::vector<double>& tb = cart2sph(l);
+        c2s.insert(c2s.end(), tb.begin(), tb.end());
+    }
+    int bnt = 0, bncol = 0;
+    double bh = 0.0, btmax = 0.0;
+    const double* btab = boys_grid(&bnt, &bncol, &bh, &btmax);
+    auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+        b.get<char>(std::max<size_t>(bytes, 8));
+        if (bytes) hcheck(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "upload integral tables");
+    };
+    up(t.ps, ps.data(), ps.size() * sizeof(GpuShell));
+    up(t.as, as.data(), as.size() * sizeof(GpuShell));
+    up(t.pe, pe.data(), pe.size() * sizeof(double));
+    up(t.pc, pc.data(), pc.size() * sizeof(double));
+    up(t.ae, ae.data(), ae.size() * sizeof(double));
+    up(t.ac, ac.data(), ac.size() * sizeof(double));
+    up(t.c2s, c2s.data(), c2s.size() * sizeof(double));
+    up(t.boys, btab, static_cast<size_t>(bnt) * bncol * sizeof(double));
+    a.prim = t.ps.as<GpuShell>();
+    a.aux = t.as.as<GpuShell>();
+    a.pexp = t.pe.as<double>();
+    a.pcoef = t.pc.as<double>();
+    a.aexp = t.ae.as<double>();
+    a.acoef = t.ac.as<double>();
+    a.c2s = t.c2s.as<double>();
+    a.prim_pure = pb.shells.empty() ? 1 : (pb.shells[0].pure ? 1 : 0);
+    a.aux_pure = (!ab || ab->shells.empty()) ? 1 : (ab->shells[0].pure ? 1 : 0);
+    a.boys.table = t.boys.as<double>();
+    a.boys.ncol = bncol;
+    a.boys.h = bh;
+    a.boys.inv_h = 1.0 / bh;
+    a.boys.tmax = btmax;
+    a.nbf = pb.nbf;
+}
+
+// The Schwarz diagonal |(mn|mn)| [nbf][nbf] on the device, for every shell pair M >= N of
+// dfhelper.cc:329-356 (grouped by class, the shell of higher l first).  Returns false (nothing
+// launched) with no device, for primary l > 3, or with PSI4AMD_HOST_INTS or
+// PSI4AMD_HOST_SCHWARZ set; the caller then uses the host engine.
+bool gpu_schwarz(dfjk_handle* h, double* fun_max) {
+    if (std::getenv("PSI4AMD_HOST_INTS") || std::getenv("PSI4AMD_HOST_SCHWARZ")) return false;
+    const Basis& pb = h->prim;
+    for (const Shell& sh : pb.shells)
+        if (sh.l > 3) return false;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return false;
+    }
+    ensure_stream(h);
+    const int np = static_cast<int>(pb.shells.size());
+    std::vector<std::vector<int2>> pcls(16);
+    for (int M = 0; M < np; ++M)
+        for (int N = 0; N <= M; ++N) {
+            int X = M, Y = N;
+            if (pb.shells[X].l < pb.shells[Y].l) std::swap(X, Y);
+            pcls[pb.shells[X].l * 4 + pb.shells[Y].l].push_back(make_int2(X, Y));
+        }
+    std::vector<int2> pall;
+    std::vector<size_t> poff(17, 0);
+    for (int c = 0; c < 16; ++c) {
+        poff[c] = pall.size();
+        pall.insert(pall.end(), pcls[c].begin(), pcls[c].end());
+    }
+    poff[16] = pall.size();
+    ThreeIndexArgs a;
+    IntTables tabs;
+    upload_int_tables(pb, nullptr, tabs, a);
+    DevBuf d_pairs, d_out;
+    d_pairs.get<int2>(std::max<size_t>(1, pall.size()));
+    hcheck(hipMemcpy(d_pairs.p, pall.data(), pall.size() * sizeof(int2), hipMemcpyHostToDevice), "upload pairs");
+    const size_t nn = static_cast<size_t>(pb.nbf) * pb.nbf;
+    double* out = d_out.get<double>(nn);
+    hcheck(hipMemsetAsync(out, 0, nn * sizeof(double), h->stream), "memset schwarz");
+    const int2* dp = d_pairs.as<int2>();
+    for (int la = 0; la <= 3; ++la)
+        for (int lb = 0; lb <= la; ++lb) {
+            const int c = la * 4 + lb;
+            const int npair = static_cast<int>(poff[c + 1] - poff[c]);
+            if (npair) launch_schwarz_class(a, la, lb, dp + poff[c], npair, out, h->stream);
+        }
+    hcheck(hipGetLastError(), "schwarz launch");
+    hcheck(hipMemcpyAsync(fun_max, out, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "schwarz download");
+    hcheck(hipStreamSynchronize(h->stream), "schwarz");
+    return true;
+}
+
+void run_three_index(dfjk_handle* h, const Basis& pb, const Basis& ab, const std::vector<std::vector<int2>>& pcls,
+                     const std::vector<int64_t>* dst, int64_t p_stride, double* out, double omega) {
     std::vector<std::vector<int2>> acls(5);
     for (size_t i = 0; i < ab.shells.size(); ++i)
         for (int k = 0; k < ab.shells[i].nfunc(); ++k) acls[ab.shells[i].l].push_back(make_int2(static_cast<int>(i), k));
@@ -682,52 +806,19 @@ void run_three_index(dfjk_handle* h, const Basis& pb, const Basis& ab, const std
         aall.insert(aall.end(), acls[c].begin(), acls[c].end());
     }
     aoff[5] = aall.size();
-    // solid-harmonic rows and the Boys grid of the host engine
-    std::vector<double> c2s;
     ThreeIndexArgs a;
-    for (int l = 0; l <= 4; ++l) {
-        a.c2s_off[l] = static_cast<int>(c2s.size());
-        const std::vector<double>& t = cart2sph(l);
-        c2s.insert(c2s.end(), t.begin(), t.end());
-    }
-    int bnt = 0, bncol = 0;
-    double bh = 0.0, btmax = 0.0;
-    const double* btab = boys_grid(&bnt, &bncol, &bh, &btmax);
-
-    DevBuf d_ps, d_as, d_pe, d_pc, d_ae, d_ac, d_c2s, d_boys, d_dst, d_pairs, d_afun;
+    IntTables tabs;
+    upload_int_tables(pb, &ab, tabs, a);
+    DevBuf d_dst, d_pairs, d_afun;
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
         b.get<char>(bytes);
         hcheck(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "upload integral tables");
     };
-    up(d_ps, ps.data(), ps.size() * sizeof(GpuShell));
-    up(d_as, as.data(), as.size() * sizeof(GpuShell));
-    up(d_pe, pe.data(), pe.size() * sizeof(double));
-    up(d_pc, pc.data(), pc.size() * sizeof(double));
-    up(d_ae, ae.data(), ae.size() * sizeof(double));
-    up(d_ac, ac.data(), ac.size() * sizeof(double));
-    up(d_c2s, c2s.data(), c2s.size() * sizeof(double));
-    up(d_boys, btab, static_cast<size_t>(bnt) * bncol * sizeof(double));
     if (dst) up(d_dst, dst->data(), dst->size() * sizeof(int64_t));
     up(d_pairs, pall.data(), std::max<size_t>(1, pall.size()) * sizeof(int2));
     up(d_afun, aall.data(), std::max<size_t>(1, aall.size()) * sizeof(int2));
-
-    a.prim = d_ps.as<GpuShell>();
-    a.aux = d_as.as<GpuShell>();
-    a.pexp = d_pe.as<double>();
-    a.pcoef = d_pc.as<double>();
-    a.aexp = d_ae.as<double>();
-    a.acoef = d_ac.as<double>();
-    a.c2s = d_c2s.as<double>();
-    a.prim_pure = pb.shells.empty() ? 1 : (pb.shells[0].pure ? 1 : 0);
-    a.aux_pure = ab.shells.empty() ? 1 : (ab.shells[0].pure ? 1 : 0);
-    a.boys.table = d_boys.as<double>();
-    a.boys.ncol = bncol;
-    a.boys.h = bh;
-    a.boys.inv_h = 1.0 / bh;
-    a.boys.tmax = btmax;
     a.dst = dst ? d_dst.as<int64_t>() : nullptr;
     a.p_stride = p_stride;
-    a.nbf = pb.nbf;
     a.out = out;
     a.omega = omega;
     const int2* dp = d_pairs.as<int2>();
@@ -1000,7 +1091,9 @@ int choose_nsplit(int ntile, int nbatch) {
     return std::min(best, std::max(1, nbatch));
 }
 
-// Lower-precision copies of B, made once per tensor on first use.
+// The lower-precision copy of B, made once per tensor on first use: FP32, read by J in both
+// mixed modes, by the FP32 half-transform, and by the BF16 half-transform, which splits it into
+// hi/lo as it stages each block.
 const float* ensure_B32(dfjk_handle* h) {
     if (!h->dB32.p) {
         float* p = h->dB32.get<float>(b_doubles(h));
@@ -1008,28 +1101,7 @@ const float* ensure_B32(dfjk_handle* h) {
     }
     return h->dB32.as<float>();
 }
-void ensure_Bbf(dfjk_handle* h) {
-    if (!h->dBh.p) {
-        uint16_t* hi = h->dBh.get<uint16_t>(b_doubles(h));
-        uint16_t* lo = h->dBl.get<uint16_t>(b_doubles(h));
-        launch_to_bf_split(h->dB.as<double>(), hi, lo, b_doubles(h), h->stream);
-    }
-}
-// tile-transposed BF16 pair, read by the half-transform for blocks used as (J, I)
-void ensure_BbfT(dfjk_handle* h) {
-    if (!h->dBhT.p) {
-        uint16_t* hi = h->dBhT.get<uint16_t>(b_doubles(h));
-        uint16_t* lo = h->dBlT.get<uint16_t>(b_doubles(h));
-        launch_to_bf_split_t(h->dB.as<double>(), hi, lo, b_doubles(h), h->stream);
-    }
-}
-void invalidate_copies(dfjk_handle* h) {
-    h->dB32.release();
-    h->dBh.release();
-    h->dBl.release();
-    h->dBhT.release();
-    h->dBlT.release();
-}
+void invalidate_copies(dfjk_handle* h) { h->dB32.release(); }
 
 // Device address of the scratch array on the current device, looked up once per device with
 // that device current (each GPU holds its own copy at its own address).
@@ -1251,18 +1323,13 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     // resident lower-precision copies (in core); the fixed-workspace entry point allocates
     // none, so an earlier allocating compute at this precision must have made them
     if (!h->ooc) {
-        const bool miss = (prec != DFJK_PREC_FP64 && doJ && !h->dB32.p) ||
-                          (prec == DFJK_PREC_BF16X3 && doK && (!h->dBh.p || !h->dBhT.p));
+        const bool miss = prec != DFJK_PREC_FP64 && !h->dB32.p;
         if (wk.fixed() && miss)
             fail(DFJK_ERR_STATE, "dfjk_compute_device_noalloc: the lower-precision copy of B for this precision does "
                                  "not exist yet; run one dfjk_compute_device at this precision first");
-        // J reads the FP32 copy in both mixed modes (2^-24 relative, the same bytes as the BF16
-        // pair's 2^-17): the J error of a density change would otherwise dominate its K error
-        if (prec == DFJK_PREC_FP32 || (prec == DFJK_PREC_BF16X3 && doJ)) ensure_B32(h);
-        if (prec == DFJK_PREC_BF16X3 && doK) {
-            ensure_Bbf(h);
-            ensure_BbfT(h);
-        }
+        // J reads the FP32 copy in both mixed modes (2^-24 relative; a BF16 hi + lo pair's
+        // 2^-17 would make the J error of a density change dominate its K error)
+        if (prec != DFJK_PREC_FP64) ensure_B32(h);
     }
 
     // J workspace in core: the symmetrised tile-packed density, d_Q, the d_Q partials per
@@ -1307,8 +1374,8 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             qblk = std::min(qblk, 65535);
         }
         if (h->ooc) {
-            // two FP64 stage buffers (plus their converted copies) share the budget with X
-            const size_t conv = prec == DFJK_PREC_FP32 ? 4 : prec == DFJK_PREC_BF16X3 ? 12 : 0;  // FP32, BF16 pairs
+            // two FP64 stage buffers (plus their FP32 copy in the mixed modes) share the budget with X
+            const size_t conv = prec == DFJK_PREC_FP64 ? 0 : 4;
             const size_t pq = qs * (2 * sizeof(double) + conv);
             const size_t stage_budget = std::max<size_t>(pq, std::min<size_t>(size_t(8) << 30, budget / 2));
             qblk = std::min<int>(qblk, static_cast<int>(std::max<size_t>(1, stage_budget / pq)));
@@ -1441,37 +1508,20 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         // this block's views of the tensor in every precision the kernels read
         const double* b64;
         const float* b32 = nullptr;
-        const uint16_t *bh = nullptr, *bl = nullptr, *bhT = nullptr, *blT = nullptr;
         if (!h->ooc) {
             const size_t off = static_cast<size_t>(qa) * qs;
             b64 = h->dB.as<double>() + off;
-            if (prec != DFJK_PREC_FP64 && doJ) b32 = h->dB32.as<float>() + off;
-            if (prec == DFJK_PREC_BF16X3 && doK) {
-                bh = h->dBh.as<uint16_t>() + off;
-                bl = h->dBl.as<uint16_t>() + off;
-                bhT = h->dBhT.as<uint16_t>() + off;
-                blT = h->dBlT.as<uint16_t>() + off;
-            }
+            if (prec != DFJK_PREC_FP64) b32 = h->dB32.as<float>() + off;
         } else {
             if (blk + 1 < nblocks) stage_copy(blk + 1);
             hcheck(hipStreamWaitEvent(s, h->ev_copied[blk & 1], 0), "wait copy");
             double* st = h->ws_stage[blk & 1].as<double>();
             b64 = st;
             hipEvent_t e0 = tm.begin();
-            if (prec == DFJK_PREC_FP32 || (prec == DFJK_PREC_BF16X3 && doJ)) {
+            if (prec != DFJK_PREC_FP64) {
                 float* c = wk.get<float>(h->ws_stage32, qs * qblk);
                 launch_to_f32(st, c, nel, s);
                 b32 = c;
-            }
-            if (prec == DFJK_PREC_BF16X3 && doK) {
-                uint16_t* c = wk.get<uint16_t>(h->ws_stagebf, 4 * qs * qblk);
-                const size_t cap = qs * qblk;
-                launch_to_bf_split(st, c, c + cap, nel, s);
-                launch_to_bf_split_t(st, c + 2 * cap, c + 3 * cap, nel, s);
-                bh = c;
-                bl = c + cap;
-                bhT = c + 2 * cap;
-                blT = c + 3 * cap;
             }
             tm.end(T_OTHER, e0);
         }
@@ -1511,11 +1561,10 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             } else {
                 uint16_t* xl = reinterpret_cast<uint16_t*>(XL);
                 uint16_t* xr = reinterpret_cast<uint16_t*>(XR);
-                launch_ht_bf3(g, bh, bl, bhT, blT, 0, nq, cth_l, ctl_l, ldc, nbw, npanel, xl, xl + xq * qblk, xrows,
-                              h->dZero.p, s);
+                launch_ht_bf3(g, b32, 0, nq, cth_l, ctl_l, ldc, nbw, npanel, xl, xl + xq * qblk, xrows, h->dZero.p, s);
                 if (general)
-                    launch_ht_bf3(g, bh, bl, bhT, blT, 0, nq, cth_r, ctl_r, ldc, nbw, npanel, xr, xr + xq * qblk,
-                                  xrows, h->dZero.p, s);
+                    launch_ht_bf3(g, b32, 0, nq, cth_r, ctl_r, ldc, nbw, npanel, xr, xr + xq * qblk, xrows,
+                                  h->dZero.p, s);
             }
             tm.end(T_HT, e0);
             h->n_ht += general ? 2 : 1;
@@ -2118,6 +2167,20 @@ int dfjk_get_pair_mask(dfjk_handle* h, uint8_t* mask, double* cutoff) {
     });
 }
 
+int dfjk_get_schwarz(dfjk_handle* h, double* fun_values, double* shell_values, double* max_integral,
+                     int* on_device) {
+    return guarded([&] {
+        require_state(h, false);
+        if (h->tensor_mode) fail(DFJK_ERR_STATE, "DFJK: a tensor-level handle has no screening integrals");
+        ensure_sparsity(h);
+        if (fun_values) std::memcpy(fun_values, h->schwarz_fun.data(), h->schwarz_fun.size() * sizeof(double));
+        if (shell_values)
+            std::memcpy(shell_values, h->schwarz_shell.data(), h->schwarz_shell.size() * sizeof(double));
+        if (max_integral) *max_integral = h->schwarz_max;
+        if (on_device) *on_device = h->schwarz_on_device ? 1 : 0;
+    });
+}
+
 int dfjk_get_fit_info(dfjk_handle* h, int* fitting, double* condition, int* naux_kept) {
     return guarded([&] {
         require_state(h, false);
@@ -2363,7 +2426,7 @@ int dfjk_finalize(dfjk_handle* h) {
                         &h->ws_Jpart, &h->ws_XL, &h->ws_XR,  &h->ws_P,    &h->ws_Pd, &h->ws_J,     &h->ws_K,
                         &h->ws_G,    &h->ws_sign, &h->ws_c32, &h->ws_cbf, &h->ws_cmap, &h->ws_ct,   &h->ws_red,
                         &h->ws_Y,    &h->ws_PC,  &h->ws_gam, &h->ws_dc, &h->ws_stage[0], &h->ws_stage[1],
-                        &h->ws_stage32, &h->ws_stagebf};
+                        &h->ws_stage32};
         for (DevBuf* b : ws) b->release();
         h->cmap_dev.clear();
         h->dM_ready = false;

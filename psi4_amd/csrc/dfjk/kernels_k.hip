@@ -308,21 +308,42 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 // stores (4 lanes per row) are free of bank conflicts.
 __device__ inline int kg_swz(int row, int chunk) { return row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3); }
 
+// Split 8 FP32 values into BF16 hi = RNE(x) and lo = RNE(x - hi), packed two per word
+// (v_cvt_pk_bf16_f32 rounds to nearest even, as f2bf does).
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef float f2v_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2v_t __attribute__((ext_vector_type(2)));
+typedef unsigned int v4u_t __attribute__((ext_vector_type(4)));
+__device__ inline void split_bf16x8(const v4f_t& x, const v4f_t& y, v4u_t& h, v4u_t& l) {
+    auto hi = [](float a, float b) { return __builtin_convertvector(f2v_t{a, b}, bf2v_t); };
+    auto lo = [](float a, float b, bf2v_t hb) {
+        return __builtin_convertvector(f2v_t{a, b} - __builtin_convertvector(hb, f2v_t), bf2v_t);
+    };
+    const bf2v_t h0 = hi(x[0], x[1]), h1 = hi(x[2], x[3]), h2 = hi(y[0], y[1]), h3 = hi(y[2], y[3]);
+    const bf2v_t l0 = lo(x[0], x[1], h0), l1 = lo(x[2], x[3], h1), l2 = lo(y[0], y[1], h2), l3 = lo(y[2], y[3], h3);
+    h = v4u_t{__builtin_bit_cast(unsigned int, h0), __builtin_bit_cast(unsigned int, h1),
+              __builtin_bit_cast(unsigned int, h2), __builtin_bit_cast(unsigned int, h3)};
+    l = v4u_t{__builtin_bit_cast(unsigned int, l0), __builtin_bit_cast(unsigned int, l1),
+              __builtin_bit_cast(unsigned int, l2), __builtin_bit_cast(unsigned int, l3)};
+}
+
 // =====================================================================================
 // Half-transform, 3-term BF16 split: X = Bh Ch + Bh Cl + Bl Ch with FP32 accumulation.
-// B as hi/lo BF16 copies, C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one
-// column tile (32).  WM*256 threads, workgroup tile 64*WM rows x 64*NBW columns.
-// Output X split again into hi/lo BF16 ([nq][xrows][ldc]).
+// B is read from its FP32 copy and split into hi/lo in registers before the LDS store (the
+// same 4 bytes per element as stored hi/lo copies, with no second copy of B resident); a
+// block used in the transposed orientation is read as column segments of its stored tile.
+// C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one column tile (32).  WM*256
+// threads, workgroup tile 64*WM rows x 64*NBW columns.  Output X split again into hi/lo
+// BF16 ([nq][xrows][ldc]).
 // =====================================================================================
 template <int NBW, int WM>
-__global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ht_bf3(const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl,
-                                                     const uint16_t* __restrict__ BhT, const uint16_t* __restrict__ BlT,
+__global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ht_bf3(const float* __restrict__ B,
                                                      const int* __restrict__ tidx, int nT, int ntp, int q0, int nq,
                                                      int npanel, const uint16_t* __restrict__ CTh,
                                                      const uint16_t* __restrict__ CTl,
                                                      int nbfp, int ldc, uint16_t* __restrict__ Xh,
                                                      uint16_t* __restrict__ Xl, int xrows,
-                                                     const uint16_t* __restrict__ ztile) {
+                                                     const float* __restrict__ ztile) {
     constexpr int NTHR = WM * 256;
     constexpr int RT = 2 * WM;
     constexpr int ROWS = 64 * WM;
@@ -368,23 +389,27 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     // before their LDS store, in two sets; the C vectors (a panel every workgroup re-reads,
     // mostly L2 hits) one step before.  Within a step the C loads are issued first, so the
     // store's wait for them leaves the younger A loads in flight.
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    v4u a0h, a0l, a1h, a1l;
+    typedef v4u_t v4u;
+    v4f_t a0x, a0y, a1x, a1y;  // 8 FP32 of one A row segment per set
     v4u c0h, c0l, c1h, c1l, c2h, c2l, c3h, c3l, c4h, c4l;
     // column tile of step st; steps past the end re-read the last one (their data is never used)
     auto jof = [&](int st) { return sl.J[min(st, nsteps - 1)]; };
-    auto gloadA = [&](int st, v4u& ah, v4u& al) {
+    // vector f covers row (f>>2) of this row tile's block, k = 8*(f&3)..+7
+    const int fr = f >> 2, fk = 8 * (f & 3);
+    auto gloadA = [&](int st, v4f_t& ax, v4f_t& ay) {
         const int J = jof(st);
         const int off = sl.off[J * RT + sub];
-        // a block stored as (J, I) is read from the tile-transposed copies, so the vector
-        // below is always a row segment of this row tile's block
-        const bool tr = (I0 + sub > J);
-        const uint16_t* th = off >= 0 ? (tr ? BhT : Bh) + qoff + off : ztile;
-        const uint16_t* tl = off >= 0 ? (tr ? BlT : Bl) + qoff + off : ztile;
-        // vector f covers stored-tile row (f>>2), cols 8*(f&3)..+7
-        const int o = (f >> 2) * TILE + 8 * (f & 3);
-        ah = *reinterpret_cast<const v4u*>(th + o);
-        al = *reinterpret_cast<const v4u*>(tl + o);
+        const float* t = off >= 0 ? B + qoff + off : ztile;
+        // a block stored as (J, I) (below the diagonal) is read as a column segment of its
+        // stored tile; the condition is uniform over the wave (sub = tid >> 7)
+        if (__builtin_amdgcn_readfirstlane((off >= 0 && I0 + sub > J) ? 1 : 0)) {
+            const float* c = t + fk * TILE + fr;
+            ax = v4f_t{c[0], c[TILE], c[2 * TILE], c[3 * TILE]};
+            ay = v4f_t{c[4 * TILE], c[5 * TILE], c[6 * TILE], c[7 * TILE]};
+        } else {
+            ax = *reinterpret_cast<const v4f_t*>(t + fr * TILE + fk);
+            ay = *reinterpret_cast<const v4f_t*>(t + fr * TILE + fk + 4);
+        }
     };
     auto gloadC = [&](int st) {
         const size_t kb = static_cast<size_t>(jof(st)) * TILE;
@@ -403,11 +428,13 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         if (NSLOT > 3) cload(3, c3h, c3l);
         if (NSLOT > 4) cload(4, c4h, c4l);
     };
-    auto lstore = [&](int buf, const v4u& ah, const v4u& al) {
+    auto lstore = [&](int buf, const v4f_t& ax, const v4f_t& ay) {
         uint16_t* Ah = Ls[buf];
         uint16_t* Al = Ah + A_SZ;
         uint16_t* Ch = Al + A_SZ;
         uint16_t* Cl = Ch + C_SZ;
+        v4u ah, al;
+        split_bf16x8(ax, ay, ah, al);
         const int ao = kg_swz(sub * 32 + (f >> 2), f & 3);
         *reinterpret_cast<v4u*>(Ah + ao) = ah;
         *reinterpret_cast<v4u*>(Al + ao) = al;
@@ -481,22 +508,22 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     // stores land in a buffer no later compute reads).
     auto mainloop = [&](auto ni) {
         gloadC(0);
-        gloadA(0, a0h, a0l);
-        lstore(0, a0h, a0l);
-        gloadA(1, a1h, a1l);
+        gloadA(0, a0x, a0y);
+        lstore(0, a0x, a0y);
+        gloadA(1, a1x, a1y);
         __syncthreads();
         int st = 0;
         // (scheduling fences keep the loads ahead of the MFMAs they are meant to overlap)
         for (; st + 1 < nsteps; st += 2) {
             gloadC(st + 1);
-            gloadA(st + 2, a0h, a0l);
+            gloadA(st + 2, a0x, a0y);
             __builtin_amdgcn_sched_barrier(0);
-            compute(0, ni, [&] { lstore(1, a1h, a1l); });
+            compute(0, ni, [&] { lstore(1, a1x, a1y); });
             __syncthreads();
             gloadC(st + 2);
-            gloadA(st + 3, a1h, a1l);
+            gloadA(st + 3, a1x, a1y);
             __builtin_amdgcn_sched_barrier(0);
-            compute(1, ni, [&] { lstore(0, a0h, a0l); });
+            compute(1, ni, [&] { lstore(0, a0x, a0y); });
             __syncthreads();
         }
         if (st < nsteps) compute(0, ni, [] {});  // odd step count: the last step is in buffer 0
@@ -861,32 +888,6 @@ __global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst
         dst[i] = static_cast<float>(src[i]);
 }
 
-__global__ void k_to_bf_split(const double* __restrict__ src, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
-                              size_t n) {
-    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const float x = static_cast<float>(src[i]);
-        const uint16_t h = f2bf(x);
-        hi[i] = h;
-        lo[i] = f2bf(x - bf2f(h));
-    }
-}
-
-// Same split, each 32x32 tile transposed: out[t][r][c] = split(src[t][c][r]).  The half-transform
-// reads these for blocks it needs in the transposed orientation, so every LDS store is a
-// 16-byte row segment.
-__global__ void k_to_bf_split_t(const double* __restrict__ src, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
-                                size_t n) {
-    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const size_t e = i % TILE2, base = i - e;
-        const float x = static_cast<float>(src[base + (e % TILE) * TILE + e / TILE]);
-        const uint16_t h = f2bf(x);
-        hi[i] = h;
-        lo[i] = f2bf(x - bf2f(h));
-    }
-}
-
 // C [nbf][ld] FP64 -> padded [nbfp][ldc] in T, column c taken from source column cmap[c] (< 0: zero)
 template <typename T>
 __global__ void k_pack_cmap(const double* __restrict__ src, int ldsrc, int nbf, const int* __restrict__ cmap,
@@ -999,16 +1000,15 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
 void set_ht_rowwaves(int wm) { g_ht_wm = (wm == 1 || wm == 2) ? wm : 0; }
 
 template <int WM>
-static void ht_bf3_wm(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, const uint16_t* BhT,
-                      const uint16_t* BlT, int q0, int nq, const uint16_t* CTh,
-                      const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
-                      const void* zt, hipStream_t s) {
+static void ht_bf3_wm(const TileGeom& g, const float* B, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
+                      int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows, const void* zt,
+                      hipStream_t s) {
     dim3 grid(xcd_grid(nq, xrows / (64 * WM) * npanel));
-    const uint16_t* z = static_cast<const uint16_t*>(zt);
+    const float* z = static_cast<const float*>(zt);
     switch (nbw) {
 #define P4A_BF3(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_ht_bf3<n, WM>), grid, dim3(WM * 256), 0, s, Bh, Bl, BhT, BlT, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, \
+        hipLaunchKernelGGL((k_ht_bf3<n, WM>), grid, dim3(WM * 256), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, \
                            CTh, CTl, g.nbfp, ldc, Xh, Xl, xrows, z);                                                \
         break;
         P4A_BF3(1) P4A_BF3(2) P4A_BF3(3) P4A_BF3(4) P4A_BF3(5)
@@ -1016,18 +1016,17 @@ static void ht_bf3_wm(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl,
     }
 }
 
-void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, const uint16_t* BhT,
-                   const uint16_t* BlT, int q0, int nq, const uint16_t* CTh,
-                   const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
-                   const void* zt, hipStream_t s) {
+void launch_ht_bf3(const TileGeom& g, const float* B, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
+                   int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows, const void* zt,
+                   hipStream_t s) {
     require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 && g.nbfp % 32 == 0 &&
                 xrows == (g.nT + 3) / 4 * 128 && g.nT <= kMaxColTiles && zt != nullptr,
             "ht_bf3 shape");
     const int wm = g_ht_wm ? g_ht_wm : 2;
     if (wm == 1)
-        ht_bf3_wm<1>(g, Bh, Bl, BhT, BlT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+        ht_bf3_wm<1>(g, B, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
     else
-        ht_bf3_wm<2>(g, Bh, Bl, BhT, BlT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+        ht_bf3_wm<2>(g, B, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
 }
 
 static void kgemm_check(int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, int ntile, int ks) {
@@ -1063,13 +1062,6 @@ void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh
 
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n);
-}
-void launch_to_bf_split(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_to_bf_split, dim3(nblocks(n, 256)), dim3(256), 0, s, src, hi, lo, n);
-}
-void launch_to_bf_split_t(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s) {
-    require(n % TILE2 == 0, "to_bf_split_t");
-    hipLaunchKernelGGL(k_to_bf_split_t, dim3(nblocks(n, 256)), dim3(256), 0, s, src, hi, lo, n);
 }
 void launch_pack_cmap_f64(const double* src, int ldsrc, int nbf, const int* cmap, double* dst, int nbfp, int ldc,
                           hipStream_t s) {

@@ -21,11 +21,11 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
                     float* X, int xrows, const void* zt, hipStream_t s);
 // Row waves per half-transform workgroup: 1 (256 threads) or 2 (512 threads); 0 = default.
 void set_ht_rowwaves(int wm);
-// 3-term BF16: B hi/lo [naux][ntp][32][32], C^T hi/lo [ldc][nbfp], X hi/lo [nq][xrows][ldc].
-void launch_ht_bf3(const TileGeom& g, const uint16_t* Bh, const uint16_t* Bl, const uint16_t* BhT,
-                   const uint16_t* BlT, int q0, int nq, const uint16_t* CTh,
-                   const uint16_t* CTl, int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows,
-                   const void* zt, hipStream_t s);
+// 3-term BF16: B FP32 [naux][ntp][32][32] (split into hi/lo as it is staged), C^T hi/lo
+// [ldc][nbfp], X hi/lo [nq][xrows][ldc].
+void launch_ht_bf3(const TileGeom& g, const float* B, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
+                   int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows, const void* zt,
+                   hipStream_t s);
 
 // P[split][tile] (+)= alpha * sum_b A_b[:, kbeg:kend] B_b[:, kbeg:kend]^T over 128x128 tiles;
 // A_b = A + b*stride, rows of length ld, xrows rows (a multiple of 128).
@@ -42,9 +42,6 @@ void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh
 
 // Precision copies and operand packing (cmap: destination column -> source column or -1).
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
-void launch_to_bf_split(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s);
-// hi/lo split with every 32x32 tile transposed (the blocks the half-transform reads as (J,I)).
-void launch_to_bf_split_t(const double* src, uint16_t* hi, uint16_t* lo, size_t n, hipStream_t s);
 void launch_pack_cmap_f64(const double* src, int ldsrc, int nbf, const int* cmap, double* dst, int nbfp, int ldc,
                           hipStream_t s);
 void launch_pack_cmap_f32(const double* src, int ldsrc, int nbf, const int* cmap, float* dst, int nbfp, int ldc,

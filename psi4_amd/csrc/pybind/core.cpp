@@ -65,6 +65,7 @@ class JK {
         int nbf = 0;
         check(dfjk_get_info(h, &nbf, nullptr, nullptr, nullptr));
         auto jk = std::make_shared<JK>(h, nbf, "MemDFJK");
+        jk->nshell_ = p.desc.nshell;
         jk->set_do_wK(do_wK);
         jk->set_memory(doubles);
         return jk;
@@ -92,6 +93,16 @@ class JK {
         py::array_t<uint8_t> m({static_cast<py::ssize_t>(nbf_), static_cast<py::ssize_t>(nbf_)});
         check(dfjk_get_pair_mask(h_, m.mutable_data(), nullptr));
         return m;
+    }
+    // screening integrals: (|(mn|mn)| [nbf][nbf], shell maxima [nshell][nshell], max, on_device)
+    py::tuple schwarz_values() {
+        double mx = 0.0;
+        int dev = 0;
+        check(dfjk_get_schwarz(h_, nullptr, nullptr, &mx, &dev));
+        const py::ssize_t n = static_cast<py::ssize_t>(nbf_), ns = static_cast<py::ssize_t>(nshell_);
+        py::array_t<double> f({n, n}), sh({ns, ns});
+        check(dfjk_get_schwarz(h_, f.mutable_data(), sh.mutable_data(), nullptr, nullptr));
+        return py::make_tuple(f, sh, mx, dev != 0);
     }
     // declared structure of non-symmetric pairs: "DETECT", "GENERAL" or "CROSS"
     void set_pair_form(const std::string& f) {
@@ -397,6 +408,7 @@ class JK {
     }
     dfjk_handle* h_;
     int nbf_;
+    int nshell_ = 0;  // primary shells (basis-built handles)
     int aux_q0_ = 0, aux_q1_ = 0;
     std::string name_;
     py::list Cl_, Cr_;
@@ -424,6 +436,7 @@ PYBIND11_MODULE(_core, m) {
         .def("memory_estimate", &JK::memory_estimate)
         .def("device_footprint", &JK::device_footprint)
         .def("pair_mask", &JK::pair_mask)
+        .def("schwarz_values", &JK::schwarz_values)
         .def("set_pair_form", &JK::set_pair_form)
         .def("initialize", &JK::initialize)
         .def("set_print", &JK::set_print)

@@ -1,4 +1,5 @@
-"""Three-index integrals (P|mn) on the device against the host engine.
+"""Integrals on the device against the host engine: three-index (P|mn), the metric (P|Q) and its
+factor, and the Schwarz diagonal (mn|mn) of the screening.
 
 The metric factor L^-1 (J = L L^T) from the device's blocked Cholesky and block forward
 substitution gives the same fitted tensor as the host factorisation ((H2O)_16, naux 1856 in
@@ -10,10 +11,12 @@ the metric by its condition number, about 1e7 here).
 
 The fitted tensor B = L^-1 (P|mn) built from the device integrals equals the one built from
 the host engine's integrals (same host metric, same device fit) to 1e-12 of the largest
-element, for water and
-(H2O)_8 in cc-pVDZ / cc-pVDZ-jkfit (s..d | s..f) and water in cc-pVTZ / cc-pVTZ-jkfit
-(s..f | s..g), and likewise the attenuated tensor of wK (omega = 0.3).  The host engine's
-integrals are themselves pinned by psi4's energies and gradients.
+element, for water and (H2O)_8 in cc-pVDZ / cc-pVDZ-jkfit (s..d | s..f) and water in
+cc-pVTZ / cc-pVTZ-jkfit (s..f | s..g), and likewise the attenuated tensor of wK (omega = 0.3).
+The host engine's integrals are themselves pinned by psi4's energies and gradients.
+
+The Schwarz diagonal |(mn|mn)| from the device equals the host engine's to 1e-12 relative per
+element, and the screening masks built from it are the host's bit for bit.
 """
 import os
 import time
@@ -121,3 +124,63 @@ def test_device_metric_matches_host(gpu_available):
     print(f"  |B_dev_metric - B_host_metric|/max {e:.2e}; metric phase {info_d['setup_ms']['metric']:.1f} ms "
           f"(device) vs {info_h['setup_ms']['metric']:.1f} ms (host)", flush=True)
     assert e < 1e-11
+
+
+def _schwarz(mol, basis, device, puream=None):
+    from psi4_amd.basisset import BasisSet
+    from psi4_amd.core import JK
+    prim = BasisSet.build(mol, basis, puream=puream)
+    aux = BasisSet.build(mol, basis + "-jkfit", puream=puream)
+    old = os.environ.pop("PSI4AMD_HOST_SCHWARZ", None)
+    if not device:
+        os.environ["PSI4AMD_HOST_SCHWARZ"] = "1"
+    try:
+        jk = JK.build(prim, aux)
+        t0 = time.perf_counter()
+        mask = jk.pair_mask()
+        dt = time.perf_counter() - t0
+        fun, shell, mx, on_dev = jk.schwarz_values()
+        nsig = jk.info()["nsig_pairs"]
+    finally:
+        os.environ.pop("PSI4AMD_HOST_SCHWARZ", None)
+        if old is not None:
+            os.environ["PSI4AMD_HOST_SCHWARZ"] = old
+    assert on_dev == device
+    return mask, fun, shell, mx, nsig, dt
+
+
+@pytest.mark.parametrize("case", ["water-tz", "water-tz-cart", "cluster8-dz", "ar5-dz", "cluster24-dz"])
+def test_device_schwarz_matches_host(gpu_available, case):
+    """The Schwarz diagonal |(mn|mn)| of the screening (dfhelper.cc:329-356) computed on the device
+    equals the host engine's element by element (1e-12 relative, every class s..f, solid-harmonic
+    and Cartesian), and the screening built from it -- function-pair mask, pair count -- is the
+    host's bit for bit; Ar5 keeps psi4's 1962 significant pairs (tests/pytests/test_jkmemory.py:37-45)."""
+    if case.startswith("water-tz"):
+        mol, basis = M.water_zmat(0.96, 104.5), "cc-pvtz"
+    elif case == "ar5-dz":
+        mol, basis = M.Molecule.from_string("Ar 0 0 0\nAr 0 0 5\nAr 0 0 15\nAr 0 0 25\nAr 0 0 35"), "cc-pvdz"
+    elif case == "cluster8-dz":
+        mol, basis = M.water_cluster((2, 2, 2), seed=9), "cc-pvdz"
+    else:
+        mol, basis = M.water_cluster((3, 4, 2), seed=0), "cc-pvdz"
+    puream = False if case.endswith("cart") else None
+    announce(f"Schwarz diagonal {case}: device")
+    md, fd, sd, xd, nd, td = _schwarz(mol, basis, True, puream)
+    announce(f"Schwarz diagonal {case}: host")
+    mh, fh, sh, xh, nh, th = _schwarz(mol, basis, False, puream)
+    err = np.abs(fd - fh)
+    # relative error where it decides the mask (values above a tenth of the threshold
+    # cutoff^2 / max), and the closest any value comes to the threshold
+    tol = 1e-24 / xh
+    big = fh >= 0.1 * tol
+    rel = float((err[big] / fh[big]).max())
+    margin = float(np.abs(fh[big] / tol - 1.0).min())
+    print(f"  {case}: nbf {fd.shape[0]}, max |dev-host|/host {rel:.2e} over values >= tol/10, closest value "
+          f"to tol {margin:.1e} relative, max {xd:.6e}; nsig {nd} (host {nh}); "
+          f"screening {td * 1e3:.1f} ms device, {th * 1e3:.1f} ms host", flush=True)
+    assert np.all(err <= 1e-12 * fh + 1e-16 * xh)
+    assert abs(xd - xh) <= 1e-13 * xh
+    assert np.all(np.abs(sd - sh) <= 1e-12 * sh + 1e-16 * xh)
+    assert np.array_equal(md, mh) and nd == nh
+    if case == "ar5-dz":
+        assert nd == 1962
