@@ -190,7 +190,7 @@ struct dfjk_handle {
     std::vector<int> otile_of_sym, otile_of_all;
     int ntile_sym = 0, ntile_all = 0;
     // resident tensor and its lower-precision copies
-    DevBuf dB, dB32;
+    DevBuf dB, dB32, dB32T;
     DevBuf dZero;  // one zero TILE x TILE FP64 block, read in place of absent blocks
     // out of core (DFHelper AO_core false, dfhelper.cc:216-260): B lives in pinned host memory
     // and each compute streams it through two device stage buffers, Q-block by Q-block
@@ -199,7 +199,7 @@ struct dfjk_handle {
     double* hB = nullptr;               // pinned host B, b_doubles
     hipStream_t cstream = nullptr;      // copy stream (default priority, ordered by events)
     hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
-    DevBuf ws_stage[2], ws_stage32;
+    DevBuf ws_stage[2], ws_stage32, ws_stage32t;
     // range-separated exchange: Bw = M (P|erf(omega r)/r|mn), same layout and fit as B
     DevBuf dBw;
     double bw_omega = -1.0;         // omega dBw was built for (< 0: none)
@@ -1101,7 +1101,17 @@ const float* ensure_B32(dfjk_handle* h) {
     }
     return h->dB32.as<float>();
 }
-void invalidate_copies(dfjk_handle* h) { h->dB32.release(); }
+// tile-transposed FP32 copy, read by the BF16 half-transform for blocks used as (J, I)
+void ensure_B32T(dfjk_handle* h) {
+    if (!h->dB32T.p) {
+        float* p = h->dB32T.get<float>(b_doubles(h));
+        launch_to_f32_t(h->dB.as<double>(), p, b_doubles(h), h->stream);
+    }
+}
+void invalidate_copies(dfjk_handle* h) {
+    h->dB32.release();
+    h->dB32T.release();
+}
 
 // Device address of the scratch array on the current device, looked up once per device with
 // that device current (each GPU holds its own copy at its own address).
@@ -1323,13 +1333,15 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     // resident lower-precision copies (in core); the fixed-workspace entry point allocates
     // none, so an earlier allocating compute at this precision must have made them
     if (!h->ooc) {
-        const bool miss = prec != DFJK_PREC_FP64 && !h->dB32.p;
+        const bool miss =
+            (prec != DFJK_PREC_FP64 && !h->dB32.p) || (prec == DFJK_PREC_BF16X3 && doK && !h->dB32T.p);
         if (wk.fixed() && miss)
             fail(DFJK_ERR_STATE, "dfjk_compute_device_noalloc: the lower-precision copy of B for this precision does "
                                  "not exist yet; run one dfjk_compute_device at this precision first");
         // J reads the FP32 copy in both mixed modes (2^-24 relative; a BF16 hi + lo pair's
         // 2^-17 would make the J error of a density change dominate its K error)
         if (prec != DFJK_PREC_FP64) ensure_B32(h);
+        if (prec == DFJK_PREC_BF16X3 && doK) ensure_B32T(h);
     }
 
     // J workspace in core: the symmetrised tile-packed density, d_Q, the d_Q partials per
@@ -1374,8 +1386,8 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             qblk = std::min(qblk, 65535);
         }
         if (h->ooc) {
-            // two FP64 stage buffers (plus their FP32 copy in the mixed modes) share the budget with X
-            const size_t conv = prec == DFJK_PREC_FP64 ? 0 : 4;
+            // two FP64 stage buffers (plus their FP32 copies in the mixed modes) share the budget with X
+            const size_t conv = prec == DFJK_PREC_FP64 ? 0 : prec == DFJK_PREC_FP32 ? 4 : 8;
             const size_t pq = qs * (2 * sizeof(double) + conv);
             const size_t stage_budget = std::max<size_t>(pq, std::min<size_t>(size_t(8) << 30, budget / 2));
             qblk = std::min<int>(qblk, static_cast<int>(std::max<size_t>(1, stage_budget / pq)));
@@ -1507,11 +1519,12 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         const size_t nel = qs * nq;
         // this block's views of the tensor in every precision the kernels read
         const double* b64;
-        const float* b32 = nullptr;
+        const float *b32 = nullptr, *b32t = nullptr;
         if (!h->ooc) {
             const size_t off = static_cast<size_t>(qa) * qs;
             b64 = h->dB.as<double>() + off;
             if (prec != DFJK_PREC_FP64) b32 = h->dB32.as<float>() + off;
+            if (prec == DFJK_PREC_BF16X3 && doK) b32t = h->dB32T.as<float>() + off;
         } else {
             if (blk + 1 < nblocks) stage_copy(blk + 1);
             hcheck(hipStreamWaitEvent(s, h->ev_copied[blk & 1], 0), "wait copy");
@@ -1522,6 +1535,11 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
                 float* c = wk.get<float>(h->ws_stage32, qs * qblk);
                 launch_to_f32(st, c, nel, s);
                 b32 = c;
+            }
+            if (prec == DFJK_PREC_BF16X3 && doK) {
+                float* c = wk.get<float>(h->ws_stage32t, qs * qblk);
+                launch_to_f32_t(st, c, nel, s);
+                b32t = c;
             }
             tm.end(T_OTHER, e0);
         }
@@ -1561,9 +1579,9 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
             } else {
                 uint16_t* xl = reinterpret_cast<uint16_t*>(XL);
                 uint16_t* xr = reinterpret_cast<uint16_t*>(XR);
-                launch_ht_bf3(g, b32, 0, nq, cth_l, ctl_l, ldc, nbw, npanel, xl, xl + xq * qblk, xrows, h->dZero.p, s);
+                launch_ht_bf3(g, b32, b32t, 0, nq, cth_l, ctl_l, ldc, nbw, npanel, xl, xl + xq * qblk, xrows, h->dZero.p, s);
                 if (general)
-                    launch_ht_bf3(g, b32, 0, nq, cth_r, ctl_r, ldc, nbw, npanel, xr, xr + xq * qblk, xrows,
+                    launch_ht_bf3(g, b32, b32t, 0, nq, cth_r, ctl_r, ldc, nbw, npanel, xr, xr + xq * qblk, xrows,
                                   h->dZero.p, s);
             }
             tm.end(T_HT, e0);
@@ -2426,7 +2444,7 @@ int dfjk_finalize(dfjk_handle* h) {
                         &h->ws_Jpart, &h->ws_XL, &h->ws_XR,  &h->ws_P,    &h->ws_Pd, &h->ws_J,     &h->ws_K,
                         &h->ws_G,    &h->ws_sign, &h->ws_c32, &h->ws_cbf, &h->ws_cmap, &h->ws_ct,   &h->ws_red,
                         &h->ws_Y,    &h->ws_PC,  &h->ws_gam, &h->ws_dc, &h->ws_stage[0], &h->ws_stage[1],
-                        &h->ws_stage32};
+                        &h->ws_stage32, &h->ws_stage32t};
         for (DevBuf* b : ws) b->release();
         h->cmap_dev.clear();
         h->dM_ready = false;

@@ -329,15 +329,17 @@ __device__ inline void split_bf16x8(const v4f_t& x, const v4f_t& y, v4u_t& h, v4
 
 // =====================================================================================
 // Half-transform, 3-term BF16 split: X = Bh Ch + Bh Cl + Bl Ch with FP32 accumulation.
-// B is read from its FP32 copy and split into hi/lo in registers before the LDS store (the
-// same 4 bytes per element as stored hi/lo copies, with no second copy of B resident); a
-// block used in the transposed orientation is read as column segments of its stored tile.
+// B is read from its FP32 copies and split into hi/lo in registers before the LDS store (the
+// same 4 bytes per element as stored hi/lo copies); a block used in the transposed
+// orientation is read from the tile-transposed FP32 copy, so every A vector is a 32-byte
+// row segment.
 // C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one column tile (32).  WM*256
 // threads, workgroup tile 64*WM rows x 64*NBW columns.  Output X split again into hi/lo
 // BF16 ([nq][xrows][ldc]).
 // =====================================================================================
 template <int NBW, int WM>
 __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ht_bf3(const float* __restrict__ B,
+                                                     const float* __restrict__ BT,
                                                      const int* __restrict__ tidx, int nT, int ntp, int q0, int nq,
                                                      int npanel, const uint16_t* __restrict__ CTh,
                                                      const uint16_t* __restrict__ CTl,
@@ -399,17 +401,12 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     auto gloadA = [&](int st, v4f_t& ax, v4f_t& ay) {
         const int J = jof(st);
         const int off = sl.off[J * RT + sub];
-        const float* t = off >= 0 ? B + qoff + off : ztile;
-        // a block stored as (J, I) (below the diagonal) is read as a column segment of its
-        // stored tile; the condition is uniform over the wave (sub = tid >> 7)
-        if (__builtin_amdgcn_readfirstlane((off >= 0 && I0 + sub > J) ? 1 : 0)) {
-            const float* c = t + fk * TILE + fr;
-            ax = v4f_t{c[0], c[TILE], c[2 * TILE], c[3 * TILE]};
-            ay = v4f_t{c[4 * TILE], c[5 * TILE], c[6 * TILE], c[7 * TILE]};
-        } else {
-            ax = *reinterpret_cast<const v4f_t*>(t + fr * TILE + fk);
-            ay = *reinterpret_cast<const v4f_t*>(t + fr * TILE + fk + 4);
-        }
+        // a block stored as (J, I) (below the diagonal) is read from the tile-transposed
+        // copy, so the vector is always a row segment of this row tile's block
+        const bool tr = (I0 + sub > J);
+        const float* t = off >= 0 ? (tr ? BT : B) + qoff + off : ztile;
+        ax = *reinterpret_cast<const v4f_t*>(t + fr * TILE + fk);
+        ay = *reinterpret_cast<const v4f_t*>(t + fr * TILE + fk + 4);
     };
     auto gloadC = [&](int st) {
         const size_t kb = static_cast<size_t>(jof(st)) * TILE;
@@ -428,13 +425,13 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         if (NSLOT > 3) cload(3, c3h, c3l);
         if (NSLOT > 4) cload(4, c4h, c4l);
     };
-    auto lstore = [&](int buf, const v4f_t& ax, const v4f_t& ay) {
+    // the A vector split into hi/lo (done while the previous step's MFMAs run)
+    v4u sah, sal;
+    auto lstore = [&](int buf, const v4u& ah, const v4u& al) {
         uint16_t* Ah = Ls[buf];
         uint16_t* Al = Ah + A_SZ;
         uint16_t* Ch = Al + A_SZ;
         uint16_t* Cl = Ch + C_SZ;
-        v4u ah, al;
-        split_bf16x8(ax, ay, ah, al);
         const int ao = kg_swz(sub * 32 + (f >> 2), f & 3);
         *reinterpret_cast<v4u*>(Ah + ao) = ah;
         *reinterpret_cast<v4u*>(Al + ao) = al;
@@ -465,7 +462,7 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     // A wave whose 16-row fragments lie past the last row tile (last row block only) skips
     // their fragment reads and MFMAs; their accumulators stay zero.
     const int nfr = __builtin_amdgcn_readfirstlane(min(4, max(0, 2 * (nT - I0 - 2 * wm))));
-    auto compute = [&](int buf, auto ni, auto&& store_next) {
+    auto compute = [&](int buf, auto ni, auto&& mid, auto&& store_next) {
         constexpr int NI = decltype(ni)::value;
         const uint16_t* Ah = Ls[buf];
         const uint16_t* Al = Ah + A_SZ;
@@ -492,6 +489,9 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
             for (int i = 0; i < NI; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        // the next A vector's hi/lo split (24 VALU ops) runs between the MFMAs below
+        __builtin_amdgcn_sched_barrier(0);
+        mid();
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
@@ -500,6 +500,15 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
             for (int i = 0; i < NI; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        if constexpr (NI > 0) {
+            // one MFMA, then two VALU ops, twelve times; then the remaining MFMAs
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, 2 * NI * NBW - 12, 0);
+        }
         __builtin_amdgcn_sched_barrier(0);
         store_next();
     };
@@ -509,7 +518,8 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     auto mainloop = [&](auto ni) {
         gloadC(0);
         gloadA(0, a0x, a0y);
-        lstore(0, a0x, a0y);
+        split_bf16x8(a0x, a0y, sah, sal);
+        lstore(0, sah, sal);
         gloadA(1, a1x, a1y);
         __syncthreads();
         int st = 0;
@@ -518,15 +528,15 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             gloadC(st + 1);
             gloadA(st + 2, a0x, a0y);
             __builtin_amdgcn_sched_barrier(0);
-            compute(0, ni, [&] { lstore(1, a1x, a1y); });
+            compute(0, ni, [&] { split_bf16x8(a1x, a1y, sah, sal); }, [&] { lstore(1, sah, sal); });
             __syncthreads();
             gloadC(st + 2);
             gloadA(st + 3, a1x, a1y);
             __builtin_amdgcn_sched_barrier(0);
-            compute(1, ni, [&] { lstore(0, a0x, a0y); });
+            compute(1, ni, [&] { split_bf16x8(a0x, a0y, sah, sal); }, [&] { lstore(0, sah, sal); });
             __syncthreads();
         }
-        if (st < nsteps) compute(0, ni, [] {});  // odd step count: the last step is in buffer 0
+        if (st < nsteps) compute(0, ni, [] {}, [] {});  // odd step count: the last step is in buffer 0
     };
     if (nsteps > 0) {
         if (nfr == 4)
@@ -887,6 +897,15 @@ __global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
         dst[i] = static_cast<float>(src[i]);
 }
+// Same, each 32x32 tile transposed: dst[t][r][c] = src[t][c][r] (the half-transform reads these
+// for the blocks it uses in the transposed orientation).
+__global__ void k_to_f32_t(const double* __restrict__ src, float* __restrict__ dst, size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t e = i % TILE2, base = i - e;
+        dst[i] = static_cast<float>(src[base + (e % TILE) * TILE + e / TILE]);
+    }
+}
 
 // C [nbf][ld] FP64 -> padded [nbfp][ldc] in T, column c taken from source column cmap[c] (< 0: zero)
 template <typename T>
@@ -1000,7 +1019,7 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
 void set_ht_rowwaves(int wm) { g_ht_wm = (wm == 1 || wm == 2) ? wm : 0; }
 
 template <int WM>
-static void ht_bf3_wm(const TileGeom& g, const float* B, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
+static void ht_bf3_wm(const TileGeom& g, const float* B, const float* BT, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
                       int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows, const void* zt,
                       hipStream_t s) {
     dim3 grid(xcd_grid(nq, xrows / (64 * WM) * npanel));
@@ -1008,7 +1027,7 @@ static void ht_bf3_wm(const TileGeom& g, const float* B, int q0, int nq, const u
     switch (nbw) {
 #define P4A_BF3(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_ht_bf3<n, WM>), grid, dim3(WM * 256), 0, s, B, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, \
+        hipLaunchKernelGGL((k_ht_bf3<n, WM>), grid, dim3(WM * 256), 0, s, B, BT, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, \
                            CTh, CTl, g.nbfp, ldc, Xh, Xl, xrows, z);                                                \
         break;
         P4A_BF3(1) P4A_BF3(2) P4A_BF3(3) P4A_BF3(4) P4A_BF3(5)
@@ -1016,7 +1035,7 @@ static void ht_bf3_wm(const TileGeom& g, const float* B, int q0, int nq, const u
     }
 }
 
-void launch_ht_bf3(const TileGeom& g, const float* B, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
+void launch_ht_bf3(const TileGeom& g, const float* B, const float* BT, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
                    int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows, const void* zt,
                    hipStream_t s) {
     require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 && g.nbfp % 32 == 0 &&
@@ -1024,9 +1043,9 @@ void launch_ht_bf3(const TileGeom& g, const float* B, int q0, int nq, const uint
             "ht_bf3 shape");
     const int wm = g_ht_wm ? g_ht_wm : 2;
     if (wm == 1)
-        ht_bf3_wm<1>(g, B, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+        ht_bf3_wm<1>(g, B, BT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
     else
-        ht_bf3_wm<2>(g, B, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+        ht_bf3_wm<2>(g, B, BT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
 }
 
 static void kgemm_check(int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, int ntile, int ks) {
@@ -1062,6 +1081,10 @@ void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh
 
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n);
+}
+void launch_to_f32_t(const double* src, float* dst, size_t n, hipStream_t s) {
+    require(n % TILE2 == 0, "to_f32_t");
+    hipLaunchKernelGGL(k_to_f32_t, dim3(nblocks(n, 256)), dim3(256), 0, s, src, dst, n);
 }
 void launch_pack_cmap_f64(const double* src, int ldsrc, int nbf, const int* cmap, double* dst, int nbfp, int ldc,
                           hipStream_t s) {

@@ -21,9 +21,9 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
                     float* X, int xrows, const void* zt, hipStream_t s);
 // Row waves per half-transform workgroup: 1 (256 threads) or 2 (512 threads); 0 = default.
 void set_ht_rowwaves(int wm);
-// 3-term BF16: B FP32 [naux][ntp][32][32] (split into hi/lo as it is staged), C^T hi/lo
-// [ldc][nbfp], X hi/lo [nq][xrows][ldc].
-void launch_ht_bf3(const TileGeom& g, const float* B, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
+// 3-term BF16: B FP32 [naux][ntp][32][32] and its tile-transposed copy BT (split into hi/lo
+// as they are staged), C^T hi/lo [ldc][nbfp], X hi/lo [nq][xrows][ldc].
+void launch_ht_bf3(const TileGeom& g, const float* B, const float* BT, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
                    int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows, const void* zt,
                    hipStream_t s);
 
@@ -42,6 +42,8 @@ void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh
 
 // Precision copies and operand packing (cmap: destination column -> source column or -1).
 void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
+// FP32 with every 32x32 tile transposed (the blocks the BF16 half-transform reads as (J,I)).
+void launch_to_f32_t(const double* src, float* dst, size_t n, hipStream_t s);
 void launch_pack_cmap_f64(const double* src, int ldsrc, int nbf, const int* cmap, double* dst, int nbfp, int ldc,
                           hipStream_t s);
 void launch_pack_cmap_f32(const double* src, int ldsrc, int nbf, const int* cmap, float* dst, int nbfp, int ldc,
