@@ -118,8 +118,11 @@ int dfjk_set_profiling(dfjk_handle* h, int on);            /* per-kernel HIP-eve
  * psi4's JK::build_JK compares with its memory to choose MEM_DF or DISK_DF (jk.cc:206-230).  As in
  * MemDFJK, wK is not counted (the helper's do_wK is set later, in preiterations).  Needs no device. */
 int dfjk_memory_estimate(dfjk_handle* h, size_t* doubles);
-/* Doubles of device memory this handle keeps resident: B (and Bw with wK), the metric during
- * the fit, D/J/K staging and the packed C panels (no psi4 counterpart). */
+/* Doubles of device memory this handle keeps resident: B (and Bw with wK), the FP32 copies of B
+ * a compute at the current precision makes (half of B for FP32; for BF16X3 that and a
+ * tile-transposed one), the metric during the fit, D/J/K staging and the packed C panels (no
+ * psi4 counterpart).  A mixed-precision compute whose copies do not fit the free device memory
+ * fails with DFJK_ERR_MEMORY and says so. */
 int dfjk_device_footprint(dfjk_handle* h, size_t* doubles);
 /* The Schwarz function-pair mask of DFHelper::prepare_sparsity (dfhelper.cc:299-420):
  * mask[m*nbf + n] = 1 when (mn|mn) >= cutoff^2 / max (mn|mn), else 0 (nbf x nbf bytes), and the

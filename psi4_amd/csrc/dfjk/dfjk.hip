@@ -941,8 +941,16 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
 // like DFHelper::AO_core with required_core_size_ (dfhelper.cc:216-260), but against the
 // device: B goes out of core only when it and 4 GiB of workspace do not fit the free HBM.
 // (psi4's memory setting sizes host buffers; here it bounds the X workspace, not residency.)
+// Doubles' worth of the lower-precision copies of B a compute at the handle's precision keeps
+// resident: the FP32 copy (FP32 and BF16X3), and the tile-transposed FP32 copy (BF16X3).
+size_t copy_doubles(const dfjk_handle* h) {
+    if (h->precision == DFJK_PREC_FP32) return b_doubles(h) / 2;
+    if (h->precision == DFJK_PREC_BF16X3) return b_doubles(h);
+    return 0;
+}
+
 size_t resident_doubles(const dfjk_handle* h) {
-    return b_doubles(h) * (h->do_wK ? 2 : 1) + static_cast<size_t>(h->naux) * h->naux;
+    return b_doubles(h) * (h->do_wK ? 2 : 1) + static_cast<size_t>(h->naux) * h->naux + copy_doubles(h);
 }
 
 void choose_subalgorithm(dfjk_handle* h) {
@@ -1094,8 +1102,18 @@ int choose_nsplit(int ntile, int nbatch) {
 // The lower-precision copy of B, made once per tensor on first use: FP32, read by J in both
 // mixed modes, by the FP32 half-transform, and by the BF16 half-transform, which splits it into
 // hi/lo as it stages each block.
+// A copy that does not fit the free device memory is refused with a message that says so.
+void require_copy_room(size_t bytes) {
+    size_t free_b = 0, total_b = 0;
+    hcheck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (bytes > free_b)
+        fail(DFJK_ERR_MEMORY, "DFJK: the FP32 copies of B for the mixed precisions need " +
+                                  std::to_string(bytes >> 20) + " MiB more device memory, " +
+                                  std::to_string(free_b >> 20) + " MiB are free; use FP64 or OUT_OF_CORE");
+}
 const float* ensure_B32(dfjk_handle* h) {
     if (!h->dB32.p) {
+        require_copy_room(b_doubles(h) * sizeof(float));
         float* p = h->dB32.get<float>(b_doubles(h));
         launch_to_f32(h->dB.as<double>(), p, b_doubles(h), h->stream);
     }
@@ -1104,6 +1122,7 @@ const float* ensure_B32(dfjk_handle* h) {
 // tile-transposed FP32 copy, read by the BF16 half-transform for blocks used as (J, I)
 void ensure_B32T(dfjk_handle* h) {
     if (!h->dB32T.p) {
+        require_copy_room(b_doubles(h) * sizeof(float));
         float* p = h->dB32T.get<float>(b_doubles(h));
         launch_to_f32_t(h->dB.as<double>(), p, b_doubles(h), h->stream);
     }
@@ -2169,6 +2188,7 @@ int dfjk_device_footprint(dfjk_handle* h, size_t* doubles) {
         const size_t nbfp = static_cast<size_t>(round_up(h->nbf, TILE));
         size_t est = static_cast<size_t>(h->naux) * h->ntp * TILE2;  // resident B
         if (h->do_wK) est *= 2;                                      // and the attenuated Bw
+        est += copy_doubles(h);  // FP32 copies of B at the current precision
         est += static_cast<size_t>(h->naux) * h->naux;              // metric (fit only)
         est += 4 * static_cast<size_t>(h->nbf) * h->nbf;            // D, J, K, staging
         est += 2 * nbfp * 320;                                      // padded C panels

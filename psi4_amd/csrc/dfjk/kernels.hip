@@ -40,8 +40,8 @@ __global__ void k_pack_w(const double* __restrict__ D, int ldD, int nbf, const i
 }
 
 // Element loaders of the resident tensor for the J path (FP64 accumulation in every case).
-// load4 / load8 read 4 / 8 consecutive elements (i a multiple of 4 / 8) with 16-byte (or, for
-// the BF16 pair's 4-element reads, 8-byte) vector loads per lane.
+// load4 / load8 read 4 / 8 consecutive elements (i a multiple of 4 / 8) with 16-byte vector
+// loads per lane.
 typedef double d2_t __attribute__((ext_vector_type(2)));
 typedef float f4v_t __attribute__((ext_vector_type(4)));
 struct BSrcF64 {
@@ -66,28 +66,6 @@ struct BSrcF32 {
         load4(i + 4, v + 4);
     }
 };
-struct BSrcBF2 {  // B ~ hi + lo, two BF16 copies
-    const uint16_t* h;
-    const uint16_t* l;
-    __device__ static double f(uint32_t bits) { return static_cast<double>(__uint_as_float(bits)); }
-    __device__ static void add(uint32_t hw, uint32_t lw, double* v) {  // two packed BF16 of each copy
-        v[0] = f(hw << 16) + f(lw << 16);
-        v[1] = f(hw & 0xFFFF0000u) + f(lw & 0xFFFF0000u);
-    }
-    __device__ void load4(size_t i, double* v) const {
-        const uint2 a = *reinterpret_cast<const uint2*>(h + i), b = *reinterpret_cast<const uint2*>(l + i);
-        add(a.x, b.x, v);
-        add(a.y, b.y, v + 2);
-    }
-    __device__ void load8(size_t i, double* v) const {
-        const uint4 a = *reinterpret_cast<const uint4*>(h + i), b = *reinterpret_cast<const uint4*>(l + i);
-        add(a.x, b.x, v);
-        add(a.y, b.y, v + 2);
-        add(a.z, b.z, v + 4);
-        add(a.w, b.w, v + 6);
-    }
-};
-
 // d_Q partials: one wave per Q, the density tile of pair t held in registers (lane l holds
 // elements 8l..8l+7 and 512+8l..512+8l+7).  Each wave takes two Q per pass, so two tiles
 // are in flight.  grid = (ntp, ceil((q1-q0)/qblk)), 256 threads.
@@ -432,16 +410,14 @@ template <class F>
 void with_bsrc(const JSource& b, F&& f) {
     if (b.kind == JSource::F64)
         f(BSrcF64{static_cast<const double*>(b.a)});
-    else if (b.kind == JSource::F32)
-        f(BSrcF32{static_cast<const float*>(b.a)});
     else
-        f(BSrcBF2{static_cast<const uint16_t*>(b.a), static_cast<const uint16_t*>(b.b)});
+        f(BSrcF32{static_cast<const float*>(b.a)});
 }
 
 void launch_j1(const TileGeom& g, const JSource& B, const double* W, int naux, int q0, int q1, double* dpart,
                hipStream_t s) {
     require(q0 >= 0 && q1 <= naux && q0 < q1, "j1 q range");
-    require(B.a && (B.kind != JSource::BF2 || B.b), "j1 source");
+    require(B.a != nullptr, "j1 source");
     const int qblk = 64;
     dim3 grid(g.ntp, (q1 - q0 + qblk - 1) / qblk);
     with_bsrc(B, [&](auto src) {
@@ -456,7 +432,7 @@ void launch_j1_reduce(const TileGeom& g, const double* dpart, int naux, double* 
 void launch_j2(const TileGeom& g, const JSource& B, const double* d, int naux, int qchunk, double* Jpart,
                hipStream_t s) {
     require(qchunk > 0, "j2 qchunk");
-    require(B.a && (B.kind != JSource::BF2 || B.b), "j2 source");
+    require(B.a != nullptr, "j2 source");
     dim3 grid(g.ntp, (naux + qchunk - 1) / qchunk);
     with_bsrc(B, [&](auto src) {
         hipLaunchKernelGGL(k_j2<decltype(src)>, grid, dim3(256), 0, s, src, d, g.ntp, naux, qchunk, Jpart);

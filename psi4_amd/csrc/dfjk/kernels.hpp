@@ -27,9 +27,8 @@ void launch_pack_w(const TileGeom& g, const double* D, int ldD, double* W, hipSt
 // dpart[t*naux+Q] = <B_Q tile t, W tile t> for Q in [q0, q1)
 // Which copy of the resident tensor the J kernels read; every variant accumulates in FP64.
 struct JSource {
-    enum Kind { F64 = 0, F32 = 1, BF2 = 2 } kind = F64;
-    const void* a = nullptr;  // FP64 / FP32 tensor, or the BF16 hi copy
-    const void* b = nullptr;  // BF16 lo copy (BF2 only)
+    enum Kind { F64 = 0, F32 = 1 } kind = F64;
+    const void* a = nullptr;  // the FP64 tensor or its FP32 copy
 };
 void launch_j1(const TileGeom& g, const JSource& B, const double* W, int naux, int q0, int q1, double* dpart,
                hipStream_t s);

@@ -84,6 +84,15 @@ def test_jk_build_host_logic():
     jk.set_do_wK(True)                  # the attenuated tensor doubles the resident device part
     assert jk.device_footprint() - foot == 464 * jk.info()["ntile_pairs"] * 1024
     assert jk.memory_estimate() == est  # psi4's MemDFJK estimate does not count wK (MemDFJK.cc:65-69)
+    from psi4_amd.core import PREC_BF16X3, PREC_FP32, PREC_FP64
+    bq = 464 * jk.info()["ntile_pairs"] * 1024   # B's doubles
+    wk = jk.device_footprint()
+    jk.set_precision(PREC_FP32)         # the FP32 copy: half of B's FP64 bytes
+    assert jk.device_footprint() - wk == bq // 2
+    jk.set_precision(PREC_BF16X3)       # and its tile-transposed twin
+    assert jk.device_footprint() - wk == bq
+    jk.set_precision(PREC_FP64)
+    assert jk.device_footprint() == wk
     with pytest.raises(RuntimeError, match="omega"):
         jk.set_omega(-1.0)
     jk.set_wcombine(True)               # combined exchange: argument checks of the weights
