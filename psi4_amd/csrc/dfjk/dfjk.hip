@@ -1386,7 +1386,9 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     }
 
     // K set-up: output tiles, Q-splits, contraction passes, packed C
-    const bool upper = sym || cross;  // symmetric K: contract the upper block-triangle only
+    // symmetric K: contract the upper block-triangle only; the cross form contracts
+    // M = X_A X_V^T over every tile in one pass and the reduction forms K = M + M^T
+    const bool upper = sym;
     const int ntile = upper ? h->ntile_sym : h->ntile_all;
     const int2* otiles = upper ? h->d_otiles_sym.as<int2>() : h->d_otiles_all.as<int2>();
     const size_t per_q = xq * esz * (general ? 2 : 1);  // X bytes per Q
@@ -1439,9 +1441,7 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
         if (!plus.empty()) passes.push_back({0, round_up(static_cast<int>(plus.size()), kstep), 1.0, 0, 0});
         passes.push_back({kplus, kplus + round_up(static_cast<int>(minus.size()), kstep), -1.0, 0, 0});
     } else if (cross) {
-        const int kc = round_up(half, kstep);
-        passes.push_back({0, kc, 1.0, 0, static_cast<size_t>(wcross)});
-        passes.push_back({0, kc, 1.0, static_cast<size_t>(wcross), 0});
+        passes.push_back({0, round_up(half, kstep), 1.0, 0, static_cast<size_t>(wcross)});
     } else {
         passes.push_back({0, std::min(ldc, round_up(ncol, kstep)), 1.0, 0, 0});
     }
@@ -1637,7 +1637,7 @@ void compute_pair(dfjk_handle* h, const double* dCl, const double* dCr, int ncol
     if (doK) {
         hipEvent_t e2 = tm.begin();
         launch_tile_reduce(P, nsplit, ntile, upper ? h->d_otile_of_sym.as<int>() : h->d_otile_of_all.as<int>(), h->nB,
-                           upper ? 1 : 0, nbf, nbf, dK, nbf, 1.0, accumulate, s);
+                           upper ? 1 : (cross ? 2 : 0), nbf, nbf, dK, nbf, 1.0, accumulate, s);
         tm.end(T_OTHER, e2);
     }
 }

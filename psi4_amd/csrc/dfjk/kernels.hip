@@ -246,6 +246,20 @@ __global__ void k_tile_reduce(const double* __restrict__ P, int nsplit, int ntil
     const int m = blockIdx.y;
     if (n >= N) return;
     int Ib = m >> 7, Jb = n >> 7, r = m & 127, c = n & 127;
+    if (sym == 2) {
+        // out = M + M^T from the partials of every tile; (m,n) and (n,m) add the same two
+        // split sums in either order, so the result is exactly symmetric
+        const int t1 = tile_of[Ib * nB + Jb], t2 = tile_of[Jb * nB + Ib];
+        double s1 = 0.0, s2 = 0.0;
+        for (int sp = 0; sp < nsplit; ++sp) {
+            if (t1 >= 0) s1 += P[(static_cast<size_t>(sp) * ntile + t1) * 16384 + r * 128 + c];
+            if (t2 >= 0) s2 += P[(static_cast<size_t>(sp) * ntile + t2) * 16384 + c * 128 + r];
+        }
+        double* o = out + static_cast<size_t>(m) * ldo + n;
+        const double v = m <= n ? s1 + s2 : s2 + s1;
+        *o = alpha * v + (accumulate ? *o : 0.0);
+        return;
+    }
     // symmetric: read the upper triangle only (also inside diagonal tiles), so the
     // result is exactly symmetric whatever the accumulation order of the tile
     if (sym && (Ib > Jb || (Ib == Jb && r > c))) {

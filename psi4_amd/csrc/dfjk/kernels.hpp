@@ -51,8 +51,9 @@ void launch_pack_c(const double* Csrc, int ldsrc, int nbf, int ncol, double scal
 void launch_gemm_nt_f64(const double* A, size_t strideA, int lda, const double* B, size_t strideB, int ldb, int M,
                         int N, int K, int nbatch, int nsplit, const int2* d_tiles, int ntile, double* P, int accumulate,
                         hipStream_t s);
-// out[m][n] = alpha * sum_split P[split][tile(m,n)][..] (+ out if accumulate).  If sym, only
-// tiles Ib<=Jb exist and the lower triangle is read transposed.  tile_of: [nB*nB] -> tile index.
+// out[m][n] = alpha * sum_split P[split][tile(m,n)][..] (+ out if accumulate).  sym 1: only
+// tiles Ib<=Jb exist and the lower triangle is read transposed; sym 2: every tile exists and
+// out = alpha * (P + P^T), exactly symmetric.  tile_of: [nB*nB] -> tile index.
 void launch_tile_reduce(const double* P, int nsplit, int ntile, const int* d_tile_of, int nB, int sym, int M, int N,
                         double* out, int ldo, double alpha, int accumulate, hipStream_t s);
 
