@@ -333,7 +333,7 @@ def main():
         s = dt_inc
         tt = tinc
         dE_step = inc["dE_vs_fp64"]
-        ncol_ht = 2 * nocc      # C_left = [A, V]; the contraction makes two passes of nocc columns
+        ncol_ht = 2 * nocc      # C_left = [A, V]; the contraction forms X_A X_V^T (nocc columns) over every tile
     else:
         hr = results[headline]
         s = hr["s_per_build"]
@@ -356,8 +356,8 @@ def main():
                            + " (K = sum_Q X_Q X_Q^T)"),
                 "launches_per_build": nlaunch / args.steps, "ms_per_launch": round(ms_per_launch, 4),
                 "algorithmic_flops_per_launch": flops_per_launch}
-    # J path: bandwidth view (two streaming passes over packed B: FP64, or the FP32 copy /
-    # BF16 hi+lo pair in the mixed precisions)
+    # J path: bandwidth view (two streaming passes over packed B: FP64, or its FP32 copy in
+    # the mixed precisions)
     # algorithmic bytes 2 * naux * n_sym * s_B (n_sym = nbf(nbf+1)/2); the kernels stream the
     # stored 32x32 tiles, padded_factor more
     ntp = jk.info()["ntile_pairs"]
