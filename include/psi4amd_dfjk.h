@@ -206,6 +206,8 @@ int dfjk_set_aux_range(dfjk_handle* h, int q0, int q1);
  * dfjk_set_aux_range (all by default) and has (q1 - q0) x nbf x nbf elements.  dEdB is the
  * derivative with respect to each element of the dense symmetric B: a perturbation of the
  * stored pair (m,n),(n,m) together changes E by dEdB[Q][m][n] + dEdB[Q][n][m].
+ * E and dEdC also work out of core (they come from J and K); dEdB needs B in core and is
+ * refused (DFJK_ERR_STATE) out of core.
  * dfjk_energy_adjoint takes host arrays, the _device variant device arrays (E stays host). */
 int dfjk_energy_adjoint(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB);
 int dfjk_energy_adjoint_device(dfjk_handle* h, const double* C, int nocc, double* E, double* dEdC, double* dEdB);

@@ -1874,7 +1874,9 @@ void compute_wk_impl(dfjk_handle* h, int n, const double* const* Cl, const doubl
 // gam_dev (device, [q1-q0][nbf][nbf]) or, when gam_host is set, are copied to host memory.
 void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double* dEdC, double* gam_dev,
                   double* gam_host) {
-    if (h->ooc) fail(DFJK_ERR_STATE, "DFJK: the energy adjoint needs B in core (subalgorithm INCORE)");
+    // E and dE/dC come from J and K, which stream B out of core too; Gamma needs B in core
+    if (h->ooc && (gam_dev || gam_host))
+        fail(DFJK_ERR_STATE, "DFJK: dE/dB of the energy adjoint needs B in core (subalgorithm INCORE)");
     const hipStream_t s = h->stream;
     const int nbf = h->nbf, nbfp = h->nbfp;
     const size_t nn = static_cast<size_t>(nbf) * nbf;

@@ -3,7 +3,8 @@ tensor stays in pinned host memory and each compute streams it through the devic
 Q-blocks.  On (H2O)_8 / cc-pVDZ with a memory budget that forces ~20 blocks, J and K equal
 the in-core handle's (same kernels, blocked sums: 1e-13 of the largest element) for a
 symmetric pair, a general non-symmetric pair and the mixed-precision cross form, and the
-streamed B read back equals the in-core B exactly.
+streamed B read back equals the in-core B exactly.  The energy adjoint's E and dE/dC equal the
+in-core ones out of core; its dE/dB needs B in core and is refused.
 """
 import numpy as np
 import pytest
@@ -83,8 +84,15 @@ def test_out_of_core_matches_in_core(gpu_available):
     Jo, Ko = _run(ooc, np.hstack([A, V]), np.hstack([V, A]), "bf16x3")
     print(f"  bf16x3 cross: J {_rel(Jo, Jc):.1e} K {_rel(Ko, Kc):.1e}", flush=True)
     assert _rel(Jo, Jc) < 1e-12 and _rel(Ko, Kc) < 1e-6
+    # the energy adjoint: E and dE/dC come from J and K, so they work out of core; dE/dB
+    # (the size of B) needs B in core
+    announce("(H2O)_8 energy adjoint E and dE/dC, in core vs out of core")
+    Ec, dCc = core.energy_adjoint(C)[:2]
+    Eo, dCo = ooc.energy_adjoint(C)[:2]
+    print(f"  adjoint: E {abs(Eo - Ec):.1e}  dE/dC {_rel(dCo, dCc):.1e}", flush=True)
+    assert abs(Eo - Ec) < 1e-12 * abs(Ec) and _rel(dCo, dCc) < 1e-12
     with pytest.raises(RuntimeError, match="in core"):
-        ooc.energy_adjoint(C)
+        ooc.energy_adjoint(C, True)
 
 
 def test_out_of_core_rejects_device_tensor_paths(gpu_available):
