@@ -372,7 +372,15 @@ void prepare_sparsity(dfjk_handle* h) {
     fun_max.assign(static_cast<size_t>(n) * n, 0.0);
     shell_max.assign(static_cast<size_t>(ns) * ns, 0.0);
     double max_val = 0.0;
-    h->schwarz_on_device = gpu_schwarz(h, fun_max.data());
+    // the device path only speeds the same integrals up: a device without room for the
+    // nbf x nbf values leaves them to the host engine
+    try {
+        h->schwarz_on_device = gpu_schwarz(h, fun_max.data());
+    } catch (const StatusError& e) {
+        if (e.code != DFJK_ERR_MEMORY) throw;
+        (void)hipGetLastError();  // the failed allocation's status
+        h->schwarz_on_device = false;
+    }
     if (h->schwarz_on_device) {
         for (int M = 0; M < ns; ++M)
             for (int N = 0; N < ns; ++N) {

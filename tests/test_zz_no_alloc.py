@@ -1,8 +1,8 @@
 """dfjk_compute_device_noalloc through the C ABI: the same results as dfjk_compute_device,
 on two calls in a row (the second with fewer columns, so the shared scratch array holds the
-first call's contents), FP64 to 1e-11 max-abs like every FP64 parity test; the cross pair form
-in BF16x3 to the mixed tolerance; a system beyond the scratch array is refused before any work
-is queued."""
+first call's contents), FP64 to 1e-11 max-abs like every FP64 parity test; a symmetric pair in
+FP32 and the cross pair form in BF16x3 to their rounding; a system beyond the scratch array is
+refused before any work is queued."""
 import ctypes
 
 import numpy as np
@@ -86,8 +86,9 @@ def test_noalloc_general_pair(h2o8):
 
 
 def test_noalloc_bf16x3_cross_form(h2o8):
-    """The incremental (cross) form in BF16x3: the allocating call makes the BF16 copies, then
-    both entry points agree to the mixed-precision rounding (different Q-blocking)."""
+    """The incremental (cross) form in BF16x3: the allocating call makes the FP32 copies of B
+    (plain and tile-transposed), then both entry points agree to the mixed-precision rounding
+    (different Q-blocking)."""
     prim, ora, jk = h2o8
     lib = _lib()
     nbf = prim.nbf()
@@ -103,6 +104,26 @@ def test_noalloc_bf16x3_cross_form(h2o8):
             assert maxabs(Kn, Kn.T) == 0.0
             assert maxabs(Jn, Ja) <= 2e-5 * np.abs(Ja).max()
             assert maxabs(Kn, Ka) <= 2e-4 * np.abs(Ka).max()
+    finally:
+        jk.set_precision(0)
+
+
+def test_noalloc_fp32_symmetric(h2o8):
+    """FP32 products on the FP32 copy of B: both entry points agree to the FP32 rounding of
+    differently blocked sums, on two calls in a row, the second smaller."""
+    prim, ora, jk = h2o8
+    lib = _lib()
+    nbf = prim.nbf()
+    jk.set_precision(1)
+    try:
+        for i, ncol in enumerate((36, 10)):
+            C = random_orbitals(nbf, ncol, seed=61 + i)
+            announce(f"noalloc call {i + 1}: FP32 symmetric pair, {ncol} columns")
+            _, Ja, Ka = _call(lib, "dfjk_compute_device", jk.handle(), C)
+            _, Jn, Kn = _call(lib, "dfjk_compute_device_noalloc", jk.handle(), C)
+            assert maxabs(Kn, Kn.T) == 0.0
+            assert maxabs(Jn, Ja) <= 1e-6 * np.abs(Ja).max()
+            assert maxabs(Kn, Ka) <= 1e-5 * np.abs(Ka).max()
     finally:
         jk.set_precision(0)
 
