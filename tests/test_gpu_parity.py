@@ -187,6 +187,22 @@ def test_cross_incremental_h2o8(h2o8, prec):
         assert maxabs(J[0], dJ) < 5 * REL_J[prec] * np.abs(dJ).max()
 
 
+def test_empty_pair_beside_a_full_one_h2o4(h2o4):
+    """A pair with no columns (a spin with no occupied orbitals) next to a full one: psi4 gives
+    D = 0, J = 0 and skips K for nocc == 0 (dfhelper.cc:3355-3358), and the full pair is
+    unaffected; the same with a single column."""
+    mol, prim, aux, ora, jk = h2o4
+    nbf = prim.nbf()
+    C = random_orbitals(nbf, 20, seed=5)
+    c1 = random_orbitals(nbf, 1, seed=6)
+    announce("empty and one-column pairs beside a full pair")
+    D, J, K = _run(jk, [C, np.zeros((nbf, 0)), c1])
+    Do, Jo, Ko = ora.compute([C, c1])
+    assert maxabs(J[0], Jo[0]) < TOL and maxabs(K[0], Ko[0]) < TOL
+    assert np.abs(D[1]).max() == 0.0 and np.abs(J[1]).max() == 0.0 and np.abs(K[1]).max() == 0.0
+    assert maxabs(J[2], Jo[1]) < TOL and maxabs(K[2], Ko[1]) < TOL
+
+
 def test_accumulate_and_aux_range_h2o4(h2o4):
     mol, prim, aux, ora, jk = h2o4
     naux = aux.nbf()
