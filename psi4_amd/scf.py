@@ -95,38 +95,42 @@ def orthogonalizer(S, s_tolerance=1e-7):
 
 
 class _DIIS:
-    """Pulay DIIS over numpy arrays or torch tensors (host or device)."""
+    """Pulay DIIS over numpy arrays or torch tensors (host or device).  The error-vector Gram
+    matrix is kept between calls: a new vector adds one row of inner products."""
 
     def __init__(self, max_vecs):
         self.max_vecs = max_vecs
         self.F, self.err = [], []
+        self.G = np.zeros((0, 0))
 
-    def _gram(self):
-        """<err_i, err_j> for all stored vectors (device reductions, one sync)."""
-        n = len(self.err)
-        pairs = [(i, j) for i in range(n) for j in range(i, n)]
-        if torch.is_tensor(self.err[0]):
-            vals = torch.stack([torch.sum(self.err[i] * self.err[j]) for i, j in pairs]).cpu().numpy()
-        else:
-            vals = [float(np.vdot(self.err[i], self.err[j])) for i, j in pairs]
-        G = np.empty((n, n))
-        for (i, j), v in zip(pairs, vals):
-            G[i, j] = G[j, i] = v
-        return G
+    @staticmethod
+    def _dots(vs, e):
+        """<v, e> for every stored v (device reductions, one sync for tensors)."""
+        if torch.is_tensor(e):
+            return torch.stack([torch.sum(v * e) for v in vs]).cpu().numpy()
+        return np.array([float(np.vdot(v, e)) for v in vs])
 
     def add(self, F, err):
         self.F.append(_copy(F))
         self.err.append(_copy(err))
-        if len(self.F) > self.max_vecs:
-            worst = int(np.argmax(np.diag(self._gram())))
+        row = self._dots(self.err, self.err[-1])
+        n = len(self.err)
+        G = np.empty((n, n))
+        G[:n - 1, :n - 1] = self.G
+        G[n - 1, :] = row
+        G[:, n - 1] = row
+        self.G = G
+        if n > self.max_vecs:
+            worst = int(np.argmax(np.diag(self.G)))
             del self.F[worst]
             del self.err[worst]
+            self.G = np.delete(np.delete(self.G, worst, axis=0), worst, axis=1)
 
     def extrapolate(self):
         n = len(self.F)
         B = -np.ones((n + 1, n + 1))
         B[n, n] = 0.0
-        B[:n, :n] = self._gram()
+        B[:n, :n] = self.G
         rhs = np.zeros(n + 1)
         rhs[n] = -1.0
         c = np.linalg.lstsq(B, rhs, rcond=None)[0]
