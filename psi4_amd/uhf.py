@@ -7,6 +7,14 @@ DIIS of procrouting/diis.py) on the in-core DF JK: each iteration is one
 ``compute()`` with C_left = [C_a,occ, C_b,occ] (lr_symmetric), i.e. two density
 pairs through the engine at once.  As in the RHF driver, the SCF algebra (orthogonalised
 eigensolves, DIIS, orbital gradient) runs in torch FP64 on the engine's device.
+
+Incremental mixed-precision builds (``mixed_precision``) follow the RHF driver's policy per
+spin: after a full FP64 build, each iteration adds J(dD_a) + J(dD_b) and K_s(dD_s) of the
+density changes, computed as two cross-form pairs (C_left = [A_s, V_s], C_right = [V_s, A_s]
+after a Procrustes alignment of each spin's old orbitals) in one engine call; a full FP64
+build every ``incfock_full_every`` iterations, whenever the running sum of the increments'
+error bounds c (||dD_a||_F + ||dD_b||_F) would pass ``build_tol``, and for every iteration
+once the orbital gradient is below ``mixed_until``.
 """
 from __future__ import annotations
 
@@ -15,7 +23,7 @@ import torch
 
 from . import ints
 from .core import JK, PREC_FP64
-from .scf import SCFConvergenceError, _DIIS, orthogonalizer
+from .scf import MIXED_ERROR_PER_DD, SCFConvergenceError, _DIIS, _PREC, orthogonalizer
 
 
 class UHFResult:
@@ -25,7 +33,8 @@ class UHFResult:
 
 class UHF:
     def __init__(self, mol, basis="cc-pvdz", df_basis_scf=None, jk=None, e_convergence=1e-10, d_convergence=1e-8,
-                 maxiter=100, diis_max_vecs=10, guess=None, s_tolerance=1e-7):
+                 maxiter=100, diis_max_vecs=10, guess=None, s_tolerance=1e-7, mixed_precision=None,
+                 mixed_until=1e-7, incfock_full_every=8, build_tol=1e-6, jk_callback=None):
         from .basisset import BasisSet
         self.mol = mol
         self.basis = BasisSet.build(mol, basis)
@@ -40,6 +49,15 @@ class UHF:
         self.diis_max_vecs = diis_max_vecs
         self.guess = guess  # None: core Hamiltonian; or starting orbitals (nbf x nmo) for both spins
         self.s_tolerance = s_tolerance
+        if mixed_precision not in _PREC:
+            raise ValueError(f"unknown precision {mixed_precision}")
+        self.mixed = None if mixed_precision == "fp64" else mixed_precision
+        self.mixed_until = mixed_until
+        self.full_every = max(1, int(incfock_full_every))
+        self.build_tol = float(build_tol)
+        self.history = []
+        # called as cb(iteration, Ca_occ, Cb_occ, (J_a, J_b, K_a, K_b), kind) after every build
+        self.jk_callback = jk_callback
         self._jk = jk
         self.Ca = self.Cb = self.Fa = self.Fb = self.eps_a = self.eps_b = None
 
@@ -56,6 +74,37 @@ class UHF:
         if hasattr(jk, "compute_device") and torch.cuda.is_available():
             return torch.device("cuda", torch.cuda.current_device())
         return torch.device("cpu")
+
+    def _delta_build(self, pairs):
+        """J and K of the density changes of both spins, one engine call with two cross-form
+        pairs: pairs = [(C_new, C_old, S C_new)] per spin; returns dJ_a, dJ_b, dK_a, dK_b."""
+        jk = self.jk()
+        lefts, rights = [], []
+        for Cn, Co, SCn in pairs:
+            if Cn.shape[1] == 0:
+                lefts.append(Cn.contiguous())
+                rights.append(Cn.contiguous())
+                continue
+            u, _, vt = np.linalg.svd((Co.T @ SCn).cpu().numpy())   # orthogonal Procrustes rotation
+            Col = Co @ torch.from_numpy(u @ vt).to(Co.device)       # same density, aligned with Cn
+            V = Cn - Col
+            A = 0.5 * (Cn + Col)
+            lefts.append(torch.cat([A, V], 1).contiguous())
+            rights.append(torch.cat([V, A], 1).contiguous())
+        jk.set_precision(_PREC[self.mixed])
+        jk.set_pair_form("CROSS")
+        try:
+            n = lefts[0].shape[0]
+            out = [torch.empty((n, n), dtype=torch.float64, device=lefts[0].device) for _ in range(4)]
+            torch.cuda.synchronize()           # C ready before the engine's stream reads it
+            jk.compute_device([L.data_ptr() for L in lefts], [R.data_ptr() for R in rights],
+                              [L.shape[1] for L in lefts], [], [out[0].data_ptr(), out[1].data_ptr()],
+                              [out[2].data_ptr(), out[3].data_ptr()], False)
+            torch.cuda.synchronize()
+            return out
+        finally:
+            jk.set_precision(PREC_FP64)
+            jk.set_pair_form("DETECT")
 
     def _build(self, Ca_occ, Cb_occ):
         """J_a, J_b, K_a, K_b of the two occupied sets, one compute() with two pairs."""
@@ -100,10 +149,33 @@ class UHF:
         diis = _DIIS(self.diis_max_vecs)
         E_old = 0.0
         na, nb = self.nalpha, self.nbeta
+        mixed_on = self.mixed is not None and dev.type == "cuda"
+        prev = None                   # (Ca_o, Cb_o, Da, Db) of the last build
+        Ja = Jb = Ka = Kb = None
+        since_full, err_bound = 0, 0.0
+        self.history = []
         for it in range(1, self.maxiter + 1):
             Ca_o, Cb_o = Ca[:, :na], Cb[:, :nb]
             Da, Db = Ca_o @ Ca_o.T, Cb_o @ Cb_o.T
-            Ja, Jb, Ka, Kb = self._build(Ca_o, Cb_o)
+            bound = None
+            if prev is not None and mixed_on and since_full + 1 < self.full_every:
+                bound = MIXED_ERROR_PER_DD[self.mixed] * (float(torch.linalg.norm(Da - prev[2])) +
+                                                          float(torch.linalg.norm(Db - prev[3])))
+                if err_bound + bound > self.build_tol:
+                    bound = None          # this increment could exceed the per-build bound
+            if bound is None:
+                Ja, Jb, Ka, Kb = self._build(Ca_o, Cb_o)
+                kind, since_full, err_bound = "fp64", 0, 0.0
+            else:
+                dJa, dJb, dKa, dKb = self._delta_build([(Ca_o, prev[0], S @ Ca_o), (Cb_o, prev[1], S @ Cb_o)])
+                Ja, Jb, Ka, Kb = Ja + dJa, Jb + dJb, Ka + dKa, Kb + dKb
+                kind = self.mixed
+                since_full += 1
+                err_bound += bound
+            prev = (Ca_o, Cb_o, Da, Db)
+            if self.jk_callback is not None:
+                self.jk_callback(it, Ca_o.cpu().numpy(), Cb_o.cpu().numpy(),
+                                 tuple(x.cpu().numpy() for x in (Ja, Jb, Ka, Kb)), kind)
             Jt = Ja + Jb
             Fa = H + Jt - Ka
             Fb = H + Jt - Kb
@@ -113,10 +185,14 @@ class UHF:
             gb = X.T @ (FDSb - FDSb.T) @ X
             err = torch.cat([ga.reshape(-1), gb.reshape(-1)])
             d_rms = float(torch.sqrt(torch.mean(err * err)))
-            if abs(E - E_old) < self.e_conv and d_rms < self.d_conv:
+            self.history.append(dict(iteration=it, energy=E, d_rms=d_rms, build=kind, err_bound=err_bound))
+            # convergence is only accepted on an FP64 build
+            if abs(E - E_old) < self.e_conv and d_rms < self.d_conv and kind == "fp64":
                 h = [x.cpu().numpy() for x in (Ca, Cb, Fa, Fb, eps_a, eps_b, Da, Db)]
                 self.Ca, self.Cb, self.Fa, self.Fb, self.eps_a, self.eps_b = h[:6]
                 return UHFResult(energy=E, Ca=h[0], Cb=h[1], Da=h[6], Db=h[7], iterations=it)
+            if mixed_on and d_rms < self.mixed_until:
+                mixed_on = False          # FP64 finishing iterations from here on
             diis.add(torch.stack([Fa, Fb]), err)
             Fx = diis.extrapolate()
             eps_a, Ca = diag(Fx[0])

@@ -130,6 +130,48 @@ def test_h2o8_incremental_mixed_scf(gpu_available, prec):
     assert max(worst) <= 1e-6
 
 
+def test_uhf_incremental_mixed_scf(gpu_available):
+    """UHF with incremental BF16x3 builds (two cross-form pairs per call): the triplet water of
+    psi4 scf6 converges to the FP64 UHF energy (1e-9 Eh at e_convergence 1e-12), mixed builds
+    were used and the last build is FP64, and every mixed build's two-electron energy
+    1/2 [tr(D_t J_t) - tr(D_a K_a) - tr(D_b K_b)] is within 1e-6 Eh of the FP64 build of the
+    same orbitals."""
+    import psi4_amd.molecule as M
+    from psi4_amd.scf import RHF
+    from psi4_amd.uhf import UHF
+    xyz = np.array([[0.0, 0.0, -0.068516219310], [0.0, -0.790689573744, 0.543701060724],
+                    [0.0, 0.790689573744, 0.543701060724]]) / 0.52917720859
+    announce("scf6 triplet water UHF, incremental bf16x3 builds")
+    guess = RHF(M.Molecule(["O", "H", "H"], xyz)).compute_energy().C
+    mol = M.Molecule(["O", "H", "H"], xyz, multiplicity=3)
+    ref = UHF(mol, e_convergence=1e-12, d_convergence=1e-10, guess=guess).compute_energy()
+    worst = []
+    solver = UHF(mol, e_convergence=1e-12, d_convergence=1e-10, guess=guess, mixed_precision="bf16x3")
+
+    def check(it, Ca, Cb, JK, kind):
+        if kind == "fp64":
+            return
+        jk = solver.jk()
+        jk.C_clear()
+        jk.C_left_add(Ca)
+        jk.C_left_add(Cb)
+        jk.compute()
+        Da, Db = Ca @ Ca.T, Cb @ Cb.T
+
+        def e2(Ja, Jb, Ka, Kb):
+            return 0.5 * float(np.sum((Da + Db) * (Ja + Jb)) - np.sum(Da * Ka) - np.sum(Db * Kb))
+        J, K = jk.J(), jk.K()
+        worst.append(abs(e2(*JK) - e2(J[0], J[1], K[0], K[1])))
+        print(f"  iteration {it} ({kind}): |dE_build| {worst[-1]:.2e}", flush=True)
+    solver.jk_callback = check
+    res = solver.compute_energy()
+    kinds = [h["build"] for h in solver.history]
+    print(f"  E {res.energy:.12f} vs FP64 {ref.energy:.12f}; builds {kinds}", flush=True)
+    assert "bf16x3" in kinds and kinds[-1] == "fp64"
+    assert abs(res.energy - ref.energy) < 1e-9
+    assert len(worst) == kinds.count("bf16x3") and max(worst) <= 1e-6
+
+
 @pytest.mark.parametrize("fit", ["cholesky", "eigen"])
 def test_h2o24_fp64_dropin(gpu_available, fit):
     """C2: plain FP64 GPU DF-JK inside RHF on (H2O)_24 (nbf 576, nocc 120): every iteration's
