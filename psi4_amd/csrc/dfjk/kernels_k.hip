@@ -333,12 +333,34 @@ __device__ inline void split_bf16x8(const v4f_t& x, const v4f_t& y, v4u_t& h, v4
 // same 4 bytes per element as stored hi/lo copies); a block used in the transposed
 // orientation is read from the tile-transposed FP32 copy, so every A vector is a 32-byte
 // row segment.
-// C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one column tile (32).  WM*256
-// threads, workgroup tile 64*WM rows x 64*NBW columns.  Output X split again into hi/lo
-// BF16 ([nq][xrows][ldc]).
+// C transposed ([ldc][nbfp], k contiguous) hi/lo.  K-step = one column tile (32).  WM x WN
+// waves, workgroup tile 64*WM rows x 16*NBW*WN columns (WN = 4: one 8-wave workgroup per CU;
+// WN = 2: two 4-wave workgroups per CU, whose barriers are independent).  DC: the C panel
+// goes global -> LDS by direct loads (no staging registers, no LDS store instructions)
+// instead of through registers.  Output X split again into hi/lo BF16 ([nq][xrows][ldc]).
 // =====================================================================================
-template <int NBW, int WM>
-__global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ht_bf3(const float* __restrict__ B,
+// s_waitcnt immediate (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+// that waits until at most n vector-memory operations of this wave are outstanding
+constexpr int wait_vm(int n) { return 0x0F70 | (n & 15) | ((n >> 4) << 14); }
+__device__ inline void fence_compiler() { __asm__ __volatile__("" ::: "memory"); }
+// (the target builtins below exist only in the device pass; the host pass of a kernel
+// template that calls them directly would drop the kernel's launch stub)
+template <int N>
+__device__ inline void wait_vmem() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_waitcnt(wait_vm(N));
+#endif
+}
+// 16 bytes per lane from global memory straight into LDS: lane L of the wave writes
+// lds + 16 L bytes (lds is wave-uniform)
+__device__ inline void load_lds16(const uint16_t* src, uint16_t* lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(src), lds, 16, 0, 0);
+#endif
+}
+
+template <int NBW, int WM, int WN, bool DC>
+__global__ __launch_bounds__(WM * WN * 64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ht_bf3(const float* __restrict__ B,
                                                      const float* __restrict__ BT,
                                                      const int* __restrict__ tidx, int nT, int ntp, int q0, int nq,
                                                      int npanel, const uint16_t* __restrict__ CTh,
@@ -346,27 +368,29 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
                                                      int nbfp, int ldc, uint16_t* __restrict__ Xh,
                                                      uint16_t* __restrict__ Xl, int xrows,
                                                      const float* __restrict__ ztile) {
-    constexpr int NTHR = WM * 256;
+    constexpr int NTHR = WM * WN * 64;
     constexpr int RT = 2 * WM;
     constexpr int ROWS = 64 * WM;
-    constexpr int PAN = 64 * NBW;
+    constexpr int PAN = 16 * NBW * WN;
     constexpr int A_SZ = ROWS * 32;          // swizzled 64-byte rows (kg_swz)
     constexpr int C_SZ = PAN * 32;
-    constexpr int NAV = ROWS * 4 / NTHR;     // 16-byte A vectors per thread per array (= 1)
+    constexpr int NAV = ROWS * 4 / NTHR;     // 16-byte A vectors per thread per array
     constexpr int NCV = PAN * 4 / NTHR;      // full rounds of 16-byte C vectors per thread per array
     constexpr int NCV_R = (PAN * 4) % NTHR;  // vectors in the last partial round
-    static_assert(NAV == 1, "one A vector per thread");
+    static_assert((NAV == 1 || NAV == 2) && NAV * NTHR == ROWS * 4, "one or two A vectors per thread");
     static_assert(NCV + (NCV_R ? 1 : 0) <= 5, "C staging slots");
     static_assert(NCV_R % 4 == 0, "whole C rows per partial round");
+    static_assert(NTHR >= kMaxColTiles, "build_steps scans with one thread per column tile");
     constexpr int NSLOT = NCV + (NCV_R ? 1 : 0);  // C vector rounds per thread
-    // lanes with no C vector in the partial round store theirs into a slot of their own
-    constexpr int NDUM = NCV_R ? NTHR - NCV_R : 1;
-    __shared__ __align__(16) uint16_t Ls[2][2 * A_SZ + 2 * C_SZ];
-    __shared__ __align__(16) uint16_t Ldum[2][NDUM * 8];
+    // the two stage buffers are separate arrays, so the compiler can tell a direct load into
+    // one from the fragment reads of the other and need not wait for it before them
+    __shared__ __align__(16) uint16_t Ls0[2 * A_SZ + 2 * C_SZ];
+    __shared__ __align__(16) uint16_t Ls1[2 * A_SZ + 2 * C_SZ];
+    auto lbuf = [&](int buf) { return buf ? Ls1 : Ls0; };
     __shared__ StepList<RT> sl;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 2, wn = wave & 3;
+    const int wm = wave / WN, wn = wave % WN;
     const int nrb = xrows / ROWS;
     const XcdItem xi = xcd_map(blockIdx.x, nrb * npanel, nq);
     if (!xi.valid) return;
@@ -385,28 +409,57 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int j = 0; j < NBW; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
 
-    // A: per row tile 32x32 bf16 = 128 vectors of 8; RT row tiles -> one vector per thread per array.
-    const int sub = tid >> 7, f = tid & 127;
+    // A: per row tile 32x32 = 128 vectors of 8; RT row tiles -> NAV vectors per thread per
+    // array, vector v = tid + NTHR * n in row tile v >> 7.
     // Staging registers.  The A vectors (B_Q blocks, streamed from HBM) are loaded two steps
     // before their LDS store, in two sets; the C vectors (a panel every workgroup re-reads,
-    // mostly L2 hits) one step before.  Within a step the C loads are issued first, so the
-    // store's wait for them leaves the younger A loads in flight.
+    // mostly L2 hits) one step before, either into registers (C loads issued first in the
+    // step, so the store's wait for them leaves the younger A loads in flight) or, with DC,
+    // straight into the other LDS buffer.
     typedef v4u_t v4u;
-    v4f_t a0x, a0y, a1x, a1y;  // 8 FP32 of one A row segment per set
+    v4f_t a0[2 * NAV], a1[2 * NAV];  // 8 FP32 of one A row segment per vector, per set
     v4u c0h, c0l, c1h, c1l, c2h, c2l, c3h, c3l, c4h, c4l;
     // column tile of step st; steps past the end re-read the last one (their data is never used)
     auto jof = [&](int st) { return sl.J[min(st, nsteps - 1)]; };
-    // vector f covers row (f>>2) of this row tile's block, k = 8*(f&3)..+7
-    const int fr = f >> 2, fk = 8 * (f & 3);
-    auto gloadA = [&](int st, v4f_t& ax, v4f_t& ay) {
+    // direct C loads: the panel's 16-row groups (PAN/16 per array, hi then lo) are dealt to
+    // the waves; a load writes 16 rows of 64 bytes, lane L into 16-byte slot L, so lane L
+    // fetches the logical chunk that the swizzle of kg_swz places in that slot
+    constexpr int NWAVE = WM * WN;
+    constexpr int NRG = PAN / 16;
+    constexpr int NLD = 2 * NRG / NWAVE;
+    static_assert(!DC || NLD * NWAVE == 2 * NRG, "whole direct loads per wave");
+    auto issueC = [&](int st, int buf) {
+        const size_t kb = static_cast<size_t>(jof(st)) * TILE;
+        const int w = __builtin_amdgcn_readfirstlane(wave);
+        const int r0 = lane >> 2, pc = lane & 3;
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int idx = w + NWAVE * i, arr = idx / NRG, rg = idx % NRG;
+            const int r = rg * 16 + r0;
+            const uint16_t* src = (arr ? CTl : CTh) + static_cast<size_t>(col0 + r) * nbfp + kb + 8 * (pc ^ ((r >> 1) & 3));
+            load_lds16(src, lbuf(buf) + 2 * A_SZ + arr * C_SZ + rg * 512);
+        }
+    };
+    // vector f = v & 127 covers row (f>>2) of its row tile's block, k = 8*(f&3)..+7
+    auto gloadA = [&](int st, v4f_t(&a)[2 * NAV]) {
         const int J = jof(st);
-        const int off = sl.off[J * RT + sub];
-        // a block stored as (J, I) (below the diagonal) is read from the tile-transposed
-        // copy, so the vector is always a row segment of this row tile's block
-        const bool tr = (I0 + sub > J);
-        const float* t = off >= 0 ? (tr ? BT : B) + qoff + off : ztile;
-        ax = *reinterpret_cast<const v4f_t*>(t + fr * TILE + fk);
-        ay = *reinterpret_cast<const v4f_t*>(t + fr * TILE + fk + 4);
+#pragma unroll
+        for (int n = 0; n < NAV; ++n) {
+            const int v = tid + NTHR * n, sub = v >> 7, f = v & 127;
+            const int off = sl.off[J * RT + sub];
+            // a block stored as (J, I) (below the diagonal) is read from the tile-transposed
+            // copy, so the vector is always a row segment of this row tile's block
+            const bool tr = (I0 + sub > J);
+            const float* t = off >= 0 ? (tr ? BT : B) + qoff + off : ztile;
+            a[2 * n] = *reinterpret_cast<const v4f_t*>(t + (f >> 2) * TILE + 8 * (f & 3));
+            a[2 * n + 1] = *reinterpret_cast<const v4f_t*>(t + (f >> 2) * TILE + 8 * (f & 3) + 4);
+        }
+    };
+    // the A vectors split into hi/lo (done while the previous step's MFMAs run)
+    v4u_t sah[NAV], sal[NAV];
+    auto splitA = [&](const v4f_t(&a)[2 * NAV]) {
+#pragma unroll
+        for (int n = 0; n < NAV; ++n) split_bf16x8(a[2 * n], a[2 * n + 1], sah[n], sal[n]);
     };
     auto gloadC = [&](int st) {
         const size_t kb = static_cast<size_t>(jof(st)) * TILE;
@@ -419,37 +472,35 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
             h = *reinterpret_cast<const v4u*>(CTh + co);
             l = *reinterpret_cast<const v4u*>(CTl + co);
         };
+        if constexpr (DC) return;
         if (NSLOT > 0) cload(0, c0h, c0l);
         if (NSLOT > 1) cload(1, c1h, c1l);
         if (NSLOT > 2) cload(2, c2h, c2l);
         if (NSLOT > 3) cload(3, c3h, c3l);
         if (NSLOT > 4) cload(4, c4h, c4l);
     };
-    // the A vector split into hi/lo (done while the previous step's MFMAs run)
-    v4u sah, sal;
-    auto lstore = [&](int buf, const v4u& ah, const v4u& al) {
-        uint16_t* Ah = Ls[buf];
+    auto lstore = [&](int buf) {
+        uint16_t* Ah = lbuf(buf);
         uint16_t* Al = Ah + A_SZ;
         uint16_t* Ch = Al + A_SZ;
         uint16_t* Cl = Ch + C_SZ;
-        const int ao = kg_swz(sub * 32 + (f >> 2), f & 3);
-        *reinterpret_cast<v4u*>(Ah + ao) = ah;
-        *reinterpret_cast<v4u*>(Al + ao) = al;
-        // every loaded value has an unconditional use, so the loads stay where they are issued
+#pragma unroll
+        for (int n = 0; n < NAV; ++n) {
+            const int v = tid + NTHR * n, f = v & 127;
+            const int ao = kg_swz((v >> 7) * 32 + (f >> 2), f & 3);
+            *reinterpret_cast<v4u*>(Ah + ao) = sah[n];
+            *reinterpret_cast<v4u*>(Al + ao) = sal[n];
+        }
+        // every loaded value has an unconditional use, so the loads stay where they are issued:
+        // lanes past the end of the partial round loaded the last vector and store it again
+        // to the same place (the same bytes as its own lane's store)
         auto cstore = [&](int it, const v4u& h, const v4u& l) {
-            const int v = tid + NTHR * it;
-            uint16_t *dh, *dl;
-            if (v < PAN * 4) {
-                const int o = kg_swz(v >> 2, v & 3);
-                dh = Ch + o;
-                dl = Cl + o;
-            } else {
-                dh = &Ldum[0][(v - PAN * 4) * 8];
-                dl = &Ldum[1][(v - PAN * 4) * 8];
-            }
-            *reinterpret_cast<v4u*>(dh) = h;
-            *reinterpret_cast<v4u*>(dl) = l;
+            const int v = min(tid + NTHR * it, PAN * 4 - 1);
+            const int o = kg_swz(v >> 2, v & 3);
+            *reinterpret_cast<v4u*>(Ch + o) = h;
+            *reinterpret_cast<v4u*>(Cl + o) = l;
         };
+        if constexpr (DC) return;
         if (NSLOT > 0) cstore(0, c0h, c0l);
         if (NSLOT > 1) cstore(1, c1h, c1l);
         if (NSLOT > 2) cstore(2, c2h, c2l);
@@ -462,9 +513,9 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     // A wave whose 16-row fragments lie past the last row tile (last row block only) skips
     // their fragment reads and MFMAs; their accumulators stay zero.
     const int nfr = __builtin_amdgcn_readfirstlane(min(4, max(0, 2 * (nT - I0 - 2 * wm))));
-    auto compute = [&](int buf, auto ni, auto&& mid, auto&& store_next) {
+    auto compute = [&](int buf, auto ni, auto&& pre, auto&& mid, auto&& store_next) {
         constexpr int NI = decltype(ni)::value;
-        const uint16_t* Ah = Ls[buf];
+        const uint16_t* Ah = lbuf(buf);
         const uint16_t* Al = Ah + A_SZ;
         const uint16_t* Ch = Al + A_SZ;
         const uint16_t* Cl = Ch + C_SZ;
@@ -485,11 +536,13 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
         for (int i = 0; i < NI; ++i)
             al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(Al + kg_swz(wm * 64 + i * 16 + (lane & 15), fc)));
         __builtin_amdgcn_sched_barrier(0);
+        pre();
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
 #pragma unroll
             for (int i = 0; i < NI; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        // the next A vector's hi/lo split (24 VALU ops) runs between the MFMAs below
+        // the next A vectors' hi/lo split (24 VALU ops per vector) runs between the MFMAs below
         __builtin_amdgcn_sched_barrier(0);
         mid();
 #pragma unroll
@@ -501,13 +554,14 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
             for (int i = 0; i < NI; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
         if constexpr (NI > 0) {
-            // one MFMA, then two VALU ops, twelve times; then the remaining MFMAs
+            // one MFMA, then two VALU ops, 12 times per A vector; then the remaining MFMAs
+            constexpr int NG = 12 * NAV < 2 * NI * NBW ? 12 * NAV : 2 * NI * NBW;
 #pragma unroll
-            for (int k = 0; k < 12; ++k) {
+            for (int k = 0; k < NG; ++k) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
                 __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
             }
-            __builtin_amdgcn_sched_group_barrier(0x008, 2 * NI * NBW - 12, 0);
+            if constexpr (2 * NI * NBW > NG) __builtin_amdgcn_sched_group_barrier(0x008, 2 * NI * NBW - NG, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
         store_next();
@@ -516,27 +570,53 @@ __global__ __launch_bounds__(WM * 256) __attribute__((amdgpu_waves_per_eu(2, 2))
     // branches around its loads and stores (loads past the last step re-read it, and their
     // stores land in a buffer no later compute reads).
     auto mainloop = [&](auto ni) {
-        gloadC(0);
-        gloadA(0, a0x, a0y);
-        split_bf16x8(a0x, a0y, sah, sal);
-        lstore(0, sah, sal);
-        gloadA(1, a1x, a1y);
+        // Staged C: the step's C and A loads are issued before its fragment reads, and the A
+        // split runs between its MFMAs.  Direct C: the A loads first; the split and the direct
+        // loads after the fragment reads (a fragment read or split that follows a direct load
+        // in program order is made to wait for it), and the step ends by waiting for all its
+        // loads, which have had the step's MFMAs to land.
+        auto loads = [&](int st, v4f_t(&a)[2 * NAV]) {
+            if constexpr (!DC) gloadC(st);
+            gloadA(st + 1, a);
+        };
+        auto waitC = [&] {
+            if constexpr (DC) wait_vmem<0>();
+        };
+        if constexpr (DC) {
+            issueC(0, 0);
+            fence_compiler();
+        } else {
+            gloadC(0);
+        }
+        gloadA(0, a0);
+        splitA(a0);
+        lstore(0);
+        gloadA(1, a1);
+        waitC();
         __syncthreads();
         int st = 0;
         // (scheduling fences keep the loads ahead of the MFMAs they are meant to overlap)
         for (; st + 1 < nsteps; st += 2) {
-            gloadC(st + 1);
-            gloadA(st + 2, a0x, a0y);
+            loads(st + 1, a0);
             __builtin_amdgcn_sched_barrier(0);
-            compute(0, ni, [&] { split_bf16x8(a1x, a1y, sah, sal); }, [&] { lstore(1, sah, sal); });
+            if constexpr (DC)
+                compute(0, ni, [&] { splitA(a1); __builtin_amdgcn_sched_barrier(0); issueC(st + 1, 1); }, [] {}, [&] { lstore(1); });
+            else
+                compute(0, ni, [] {}, [&] { splitA(a1); }, [&] { lstore(1); });
+            waitC();
             __syncthreads();
-            gloadC(st + 2);
-            gloadA(st + 3, a1x, a1y);
+            loads(st + 2, a1);
             __builtin_amdgcn_sched_barrier(0);
-            compute(1, ni, [&] { split_bf16x8(a0x, a0y, sah, sal); }, [&] { lstore(0, sah, sal); });
+            if constexpr (DC)
+                compute(1, ni, [&] { splitA(a0); __builtin_amdgcn_sched_barrier(0); issueC(st + 2, 0); }, [] {}, [&] { lstore(0); });
+            else
+                compute(1, ni, [] {}, [&] { splitA(a0); }, [&] { lstore(0); });
+            waitC();
             __syncthreads();
         }
-        if (st < nsteps) compute(0, ni, [] {}, [] {});  // odd step count: the last step is in buffer 0
+        if (st < nsteps) compute(0, ni, [] {}, [] {}, [] {});  // odd step count: the last step is in buffer 0
+        // direct loads still in flight (steps past the end) land before the LDS is released
+        if constexpr (DC) wait_vmem<0>();
     };
     if (nsteps > 0) {
         if (nfr == 4)
@@ -993,7 +1073,7 @@ static void ht_mfma_wm(const TileGeom& g, const T* B, int q0, int nq, const T* C
     }
 }
 
-int g_ht_wm = 0;       // 0: default shape; 1 or 2 force the row-wave count (tuning sweeps)
+int g_ht_wm = 0;       // 0: default shape; 1, 2 (and 4 for BF16x3) select others (tuning sweeps)
 
 template <typename T>
 static void ht_mfma(const TileGeom& g, const T* B, int q0, int nq, const T* C, int ldc, int nbw, int npanel, T* X,
@@ -1016,19 +1096,21 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
                     float* X, int xrows, const void* zt, hipStream_t s) {
     ht_mfma<float>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
 }
-void set_ht_rowwaves(int wm) { g_ht_wm = (wm == 1 || wm == 2) ? wm : 0; }
+void set_ht_rowwaves(int wm) { g_ht_wm = (wm >= 1 && wm <= 4) ? wm : 0; }
 
-template <int WM>
+// npanel counts panels of 64*nbw columns; the WN = 2 shape covers each with two workgroups
+template <int WM, int WN, bool DC>
 static void ht_bf3_wm(const TileGeom& g, const float* B, const float* BT, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,
                       int ldc, int nbw, int npanel, uint16_t* Xh, uint16_t* Xl, int xrows, const void* zt,
                       hipStream_t s) {
-    dim3 grid(xcd_grid(nq, xrows / (64 * WM) * npanel));
+    const int np = npanel * 4 / WN;
+    dim3 grid(xcd_grid(nq, xrows / (64 * WM) * np));
     const float* z = static_cast<const float*>(zt);
     switch (nbw) {
 #define P4A_BF3(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_ht_bf3<n, WM>), grid, dim3(WM * 256), 0, s, B, BT, g.d_tidx, g.nT, g.ntp, q0, nq, npanel, \
-                           CTh, CTl, g.nbfp, ldc, Xh, Xl, xrows, z);                                                \
+        hipLaunchKernelGGL((k_ht_bf3<n, WM, WN, DC>), grid, dim3(WM * WN * 64), 0, s, B, BT, g.d_tidx, g.nT, g.ntp, q0, nq, \
+                           np, CTh, CTl, g.nbfp, ldc, Xh, Xl, xrows, z);                                           \
         break;
         P4A_BF3(1) P4A_BF3(2) P4A_BF3(3) P4A_BF3(4) P4A_BF3(5)
 #undef P4A_BF3
@@ -1041,11 +1123,18 @@ void launch_ht_bf3(const TileGeom& g, const float* B, const float* BT, int q0, i
     require(nbw >= 1 && nbw <= 5 && ldc == npanel * 64 * nbw && nq > 0 && nq <= 65535 && g.nbfp % 32 == 0 &&
                 xrows == (g.nT + 3) / 4 * 128 && g.nT <= kMaxColTiles && zt != nullptr,
             "ht_bf3 shape");
-    const int wm = g_ht_wm ? g_ht_wm : 2;
-    if (wm == 1)
-        ht_bf3_wm<1>(g, B, BT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+    // default: one 8-wave workgroup of 128 x 64*nbw per CU, C loaded straight into LDS.
+    // Kept for comparison: 1 (4 waves over 64 rows), 3 (4-wave workgroups of 128 x 32*nbw,
+    // two per CU: as fast, but each B block is read by twice as many workgroups, 1.3x the
+    // HBM reads), 4 (the 8-wave shape with C staged through registers).
+    if (g_ht_wm == 1)
+        ht_bf3_wm<1, 4, false>(g, B, BT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+    else if (g_ht_wm == 3)
+        ht_bf3_wm<2, 2, true>(g, B, BT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+    else if (g_ht_wm == 4)
+        ht_bf3_wm<2, 4, false>(g, B, BT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
     else
-        ht_bf3_wm<2>(g, B, BT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
+        ht_bf3_wm<2, 4, true>(g, B, BT, q0, nq, CTh, CTl, ldc, nbw, npanel, Xh, Xl, xrows, zt, s);
 }
 
 static void kgemm_check(int xrows, int ld, int kbeg, int kend, int nbatch, int nsplit, int ntile, int ks) {

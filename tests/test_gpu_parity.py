@@ -279,3 +279,47 @@ def test_mixed_two_panel_incremental_synthetic(gpu_available, prec, shape):
     assert scale_d > 1e-6 * np.abs(Kfull[0]).max()
     assert np.abs(out[prec][0] - out[0][0]).max() < 5 * REL_J[prec] * np.abs(out[0][0]).max()
     assert err < 5 * REL_K[prec] * scale_d
+
+
+@pytest.mark.parametrize("mode", ["1", "3", "4"])
+def test_half_transform_shapes_agree_bitwise_synthetic(gpu_available, mode):
+    """The BF16x3 half-transform's other workgroup shapes (PSI4AMD_HT_ROWWAVES: 1 = 4 waves
+    over 64 rows, 3 = two 4-wave 128x(32*nbw) workgroups per CU, 4 = C staged through
+    registers) run the same MFMAs in the same order for every element of X, so K equals the
+    default shape's bit for bit (two column panels of 256: nbf 420, 2*nocc = 400)."""
+    import os
+    from psi4_amd.core import JK
+    from psi4_amd.workloads import synthetic_C
+    nbf, naux, nocc = 420, 96, 200
+    C = synthetic_C(nbf, nocc, 5)
+    rng = np.random.default_rng(6)
+    Co = np.linalg.qr(C + 1e-4 * rng.standard_normal(C.shape))[0]
+    u, _, vt = np.linalg.svd(Co.T @ C)
+    Co = Co @ (u @ vt)
+    A, V = 0.5 * (C + Co), C - Co
+
+    def run(m):
+        # the shape is read when a handle is initialised and applies to every handle
+        os.environ["PSI4AMD_HT_ROWWAVES"] = m
+        jk = JK.build_tensor(nbf, naux)
+        jk.set_print(0)
+        jk.initialize()
+        jk.fill_synthetic(99)
+        jk.set_precision(2)
+        _, J, K = _run(jk, [np.hstack([A, V])], [np.hstack([V, A])])
+        return J[0].copy(), K[0].copy()
+
+    before = os.environ.get("PSI4AMD_HT_ROWWAVES")
+    try:
+        announce(f"half-transform shape {mode} vs the default, BF16x3 cross form")
+        J0, K0 = run("0")
+        J1, K1 = run(mode)
+    finally:
+        run("0")  # restores the default shape for the handles of later tests
+        if before is None:
+            del os.environ["PSI4AMD_HT_ROWWAVES"]
+        else:
+            os.environ["PSI4AMD_HT_ROWWAVES"] = before
+    assert np.abs(K0).max() > 0
+    assert np.array_equal(K1, K0)
+    assert np.array_equal(J1, J0)
