@@ -327,6 +327,7 @@ size_t memory_limited(const dfjk_handle* h, size_t want, size_t resident) {
 void upload_tiles(dfjk_handle* h) {
     hcheck(hipMemset(h->dZero.get<double>(TILE2), 0, TILE2 * sizeof(double)), "zero tile");
     if (const char* e = std::getenv("PSI4AMD_HT_ROWWAVES")) set_ht_rowwaves(std::atoi(e));
+    if (const char* e = std::getenv("PSI4AMD_KGEMM_DC")) set_kgemm_direct(std::atoi(e));
     if (const char* e = std::getenv("PSI4AMD_KGEMM_WGS")) g_kgemm_wgs = std::max(256, std::atoi(e));
     if (const char* e = std::getenv("PSI4AMD_X_BUDGET_GB"))
         g_x_budget = g_x_budget_mixed = static_cast<size_t>(std::max(1, std::atoi(e))) << 30;

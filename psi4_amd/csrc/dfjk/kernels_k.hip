@@ -833,15 +833,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // The MFMA accumulators are folded into a second set of FP32 totals every kFold32 k-terms
 // (FP64 totals would not fit beside the fragments in 256 registers): a sum of N terms then
 // rounds like two chains of about sqrt(N) additions instead of one chain of N.
+// DC: the operands go from global memory straight into LDS (global_load_lds_dwordx4, each
+// wave 8 loads of 16 rows per step, issued after the step's fragment reads) instead of
+// through staging registers.
 // =====================================================================================
 
+template <bool DC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_kgemm_bf3(
     const uint16_t* __restrict__ Ah_, const uint16_t* __restrict__ Al_, const uint16_t* __restrict__ Bh_,
     const uint16_t* __restrict__ Bl_, size_t stride, int ld, int kbeg, int kend, int nbatch, int nsplit,
     const int2* __restrict__ tiles, int ntile, double* __restrict__ P, double alpha, int accumulate, int upper,
     int nvalid) {
     constexpr int KS = 32, SZ = 128 * 32;
-    __shared__ uint16_t Ls[2][4 * SZ];
+    __shared__ __align__(16) uint16_t Ls0[4 * SZ];
+    __shared__ __align__(16) uint16_t Ls1[4 * SZ];
+    auto lbuf = [&](int b) { return b ? Ls1 : Ls0; };
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const XcdItem xi = xcd_map(blockIdx.x, ntile, nsplit);
@@ -895,7 +901,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     };
     const int o0 = kg_swz(vrow, vc), o1 = kg_swz(vrow + 64, vc);
     auto lstore = [&](int buf) {
-        uint16_t* L = Ls[buf];
+        uint16_t* L = lbuf(buf);
         *reinterpret_cast<uint4*>(L + o0) = a0h;
         *reinterpret_cast<uint4*>(L + SZ + o0) = a0l;
         *reinterpret_cast<uint4*>(L + 2 * SZ + o0) = b0h;
@@ -905,9 +911,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         *reinterpret_cast<uint4*>(L + 2 * SZ + o1) = b1h;
         *reinterpret_cast<uint4*>(L + 3 * SZ + o1) = b1l;
     };
+    // direct loads of step s into buf: the 32 groups of 16 LDS rows (Ah, Al, Bh, Bl x 8) are
+    // dealt to the waves; lane L lands in 16-byte slot L and fetches the logical chunk that
+    // the swizzle of kg_swz places there
+    auto issue = [&](int s, int buf) {
+        const int b = b0 + s / nks;
+        const int k0 = kbeg + (s % nks) * KS;
+        const int w = __builtin_amdgcn_readfirstlane(wave);
+        const int r0 = lane >> 2, pc = lane & 3;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int arr = i >> 1, rg = w + 4 * (i & 1);  // group w + 4 i of the 32
+            const int r = rg * 16 + r0;
+            const uint16_t* base = arr == 0 ? Ah_ : arr == 1 ? Al_ : arr == 2 ? Bh_ : Bl_;
+            const size_t row = static_cast<size_t>((arr < 2 ? m0 : n0) + r);
+            load_lds16(base + static_cast<size_t>(b) * stride + row * ld + k0 + 8 * (pc ^ ((r >> 1) & 3)),
+                       lbuf(buf) + arr * SZ + rg * 512);
+        }
+    };
     const int fc = lane >> 4;
-    auto compute = [&](int buf) {
-        const uint16_t* L = Ls[buf];
+    auto compute = [&](int buf, auto&& pre) {
+        const uint16_t* L = lbuf(buf);
         bf8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -917,6 +941,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             al[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + SZ + ra));
             bh[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + 2 * SZ + rb));
             bl[i] = __builtin_bit_cast(bf8_t, *reinterpret_cast<const uint4*>(L + 3 * SZ + rb));
+        }
+        if constexpr (DC) {
+            __builtin_amdgcn_sched_barrier(0);
+            pre();
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -936,21 +965,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     };
 
     if (nsteps > 0) {
-        gload(0);
-        lstore(0);
+        if constexpr (DC) {
+            issue(0, 0);
+            wait_vmem<0>();
+        } else {
+            gload(0);
+            lstore(0);
+        }
         __syncthreads();
         int buf = 0, nf = 0;
         for (int s = 0; s < nsteps; ++s) {
             const bool more = s + 1 < nsteps;
-            if (more) gload(s + 1);
+            auto next = [&] {
+                if constexpr (DC)
+                    if (more) issue(s + 1, buf ^ 1);
+            };
+            if constexpr (!DC)
+                if (more) gload(s + 1);
             if (!idle) {
-                compute(buf);
+                compute(buf, next);
                 if (++nf == kFold32 / KS) {
                     fold();
                     nf = 0;
                 }
+            } else {
+                next();
             }
-            if (more) lstore(buf ^ 1);
+            if constexpr (DC)
+                wait_vmem<0>();
+            else if (more)
+                lstore(buf ^ 1);
             __syncthreads();
             buf ^= 1;
         }
@@ -1097,6 +1141,8 @@ void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const flo
     ht_mfma<float>(g, B, q0, nq, C, ldc, nbw, npanel, X, xrows, zt, s);
 }
 void set_ht_rowwaves(int wm) { g_ht_wm = (wm >= 1 && wm <= 4) ? wm : 0; }
+int g_kgemm_dc = 1;    // BF16x3 contraction: operands loaded straight into LDS (1) or staged (0)
+void set_kgemm_direct(int on) { g_kgemm_dc = on ? 1 : 0; }
 
 // npanel counts panels of 64*nbw columns; the WN = 2 shape covers each with two workgroups
 template <int WM, int WN, bool DC>
@@ -1164,7 +1210,7 @@ void launch_kgemm_bf3(const uint16_t* Ah, const uint16_t* Al, const uint16_t* Bh
                       int ntile, double* P, double alpha, int accumulate, int upper, hipStream_t s) {
     kgemm_check(xrows, ld, kbeg, kend, nbatch, nsplit, ntile, 32);
     require(nvalid > 0 && nvalid <= xrows, "kgemm valid rows");
-    hipLaunchKernelGGL(k_kgemm_bf3, dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, Ah, Al, Bh, Bl, stride, ld, kbeg,
+    hipLaunchKernelGGL(g_kgemm_dc ? k_kgemm_bf3<true> : k_kgemm_bf3<false>, dim3(xcd_grid(nsplit, ntile)), dim3(256), 0, s, Ah, Al, Bh, Bl, stride, ld, kbeg,
                        kend, nbatch, nsplit, tiles, ntile, P, alpha, accumulate, upper, nvalid);
 }
 

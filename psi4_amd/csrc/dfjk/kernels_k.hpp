@@ -19,8 +19,12 @@ void launch_ht2_f64(const TileGeom& g, const double* B, int q0, int nq, const do
                     double* X, int xrows, const void* zt, hipStream_t s);
 void launch_ht2_f32(const TileGeom& g, const float* B, int q0, int nq, const float* C, int ldc, int nbw, int npanel,
                     float* X, int xrows, const void* zt, hipStream_t s);
-// Row waves per half-transform workgroup: 1 (256 threads) or 2 (512 threads); 0 = default.
+// Half-transform workgroup shape (tuning): 0 = default; 1 = one row-wave (256 threads);
+// 2 = two row-waves (512 threads, the FP64/FP32 default); for BF16x3 also 3 = two 4-wave
+// 128-row workgroups per CU and 4 = C staged through registers.
 void set_ht_rowwaves(int wm);
+// BF16x3 contraction: load its operands straight into LDS (1) or stage them in registers (0).
+void set_kgemm_direct(int on);
 // 3-term BF16: B FP32 [naux][ntp][32][32] and its tile-transposed copy BT (split into hi/lo
 // as they are staged), C^T hi/lo [ldc][nbfp], X hi/lo [nq][xrows][ldc].
 void launch_ht_bf3(const TileGeom& g, const float* B, const float* BT, int q0, int nq, const uint16_t* CTh, const uint16_t* CTl,

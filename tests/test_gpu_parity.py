@@ -281,16 +281,19 @@ def test_mixed_two_panel_incremental_synthetic(gpu_available, prec, shape):
     assert err < 5 * REL_K[prec] * scale_d
 
 
-@pytest.mark.parametrize("mode", ["1", "3", "4"])
-def test_half_transform_shapes_agree_bitwise_synthetic(gpu_available, mode):
+@pytest.mark.parametrize("var,val", [("PSI4AMD_HT_ROWWAVES", "1"), ("PSI4AMD_HT_ROWWAVES", "3"),
+                                     ("PSI4AMD_HT_ROWWAVES", "4"), ("PSI4AMD_KGEMM_DC", "0")])
+def test_kernel_shapes_agree_bitwise_synthetic(gpu_available, var, val):
     """The BF16x3 half-transform's other workgroup shapes (PSI4AMD_HT_ROWWAVES: 1 = 4 waves
     over 64 rows, 3 = two 4-wave 128x(32*nbw) workgroups per CU, 4 = C staged through
-    registers) run the same MFMAs in the same order for every element of X, so K equals the
-    default shape's bit for bit (two column panels of 256: nbf 420, 2*nocc = 400)."""
+    registers) and the contraction with its operands staged through registers
+    (PSI4AMD_KGEMM_DC=0) run the same MFMAs in the same order for every element, so J and K
+    equal the defaults' bit for bit (two column panels of 256: nbf 420, 2*nocc = 400)."""
     import os
     from psi4_amd.core import JK
     from psi4_amd.workloads import synthetic_C
     nbf, naux, nocc = 420, 96, 200
+    default = {"PSI4AMD_HT_ROWWAVES": "0", "PSI4AMD_KGEMM_DC": "1"}[var]
     C = synthetic_C(nbf, nocc, 5)
     rng = np.random.default_rng(6)
     Co = np.linalg.qr(C + 1e-4 * rng.standard_normal(C.shape))[0]
@@ -298,9 +301,9 @@ def test_half_transform_shapes_agree_bitwise_synthetic(gpu_available, mode):
     Co = Co @ (u @ vt)
     A, V = 0.5 * (C + Co), C - Co
 
-    def run(m):
-        # the shape is read when a handle is initialised and applies to every handle
-        os.environ["PSI4AMD_HT_ROWWAVES"] = m
+    def run(v):
+        # the setting is read when a handle is initialised and applies to every handle
+        os.environ[var] = v
         jk = JK.build_tensor(nbf, naux)
         jk.set_print(0)
         jk.initialize()
@@ -309,17 +312,17 @@ def test_half_transform_shapes_agree_bitwise_synthetic(gpu_available, mode):
         _, J, K = _run(jk, [np.hstack([A, V])], [np.hstack([V, A])])
         return J[0].copy(), K[0].copy()
 
-    before = os.environ.get("PSI4AMD_HT_ROWWAVES")
+    before = os.environ.get(var)
     try:
-        announce(f"half-transform shape {mode} vs the default, BF16x3 cross form")
-        J0, K0 = run("0")
-        J1, K1 = run(mode)
+        announce(f"{var}={val} vs the default, BF16x3 cross form")
+        J0, K0 = run(default)
+        J1, K1 = run(val)
     finally:
-        run("0")  # restores the default shape for the handles of later tests
+        run(default)  # restores the default for the handles of later tests
         if before is None:
-            del os.environ["PSI4AMD_HT_ROWWAVES"]
+            del os.environ[var]
         else:
-            os.environ["PSI4AMD_HT_ROWWAVES"] = before
+            os.environ[var] = before
     assert np.abs(K0).max() > 0
     assert np.array_equal(K1, K0)
     assert np.array_equal(J1, J0)
