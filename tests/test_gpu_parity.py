@@ -71,6 +71,24 @@ def test_symmetric_panel_widths_h2o4(h2o4, width):
     assert maxabs(K[0], Ko[0]) < TOL * max(1.0, width / 20)
 
 
+@pytest.mark.parametrize("width", [16, 70, 130, 250, 300, 330])
+def test_bf16x3_panel_widths_h2o4(h2o4, width):
+    """The BF16x3 kernels at every panel width (64-column blocks per wave 1-5, one and two
+    panels: the direct loads of the C panel are dealt to the waves by width) against the
+    oracle, relative to max|K| and max|J|."""
+    mol, prim, aux, ora, jk = h2o4
+    C = random_orbitals(prim.nbf(), width, seed=width)
+    announce(f"symmetric BF16x3 J/K width={width}")
+    jk.set_precision(2)
+    try:
+        _, J, K = _run(jk, [C])
+    finally:
+        jk.set_precision(0)
+    _, Jo, Ko = ora.compute([C])
+    assert maxabs(K[0], Ko[0]) < 5 * REL_K[2] * np.abs(Ko[0]).max()
+    assert maxabs(J[0], Jo[0]) < 5 * REL_J[2] * np.abs(Jo[0]).max()
+
+
 def test_nonsymmetric_multi_pair_h2o4(h2o4):
     mol, prim, aux, ora, jk = h2o4
     nbf = prim.nbf()
