@@ -415,7 +415,8 @@ __global__ __launch_bounds__(WM * WN * 64) __attribute__((amdgpu_waves_per_eu(2,
     // before their LDS store, in two sets; the C vectors (a panel every workgroup re-reads,
     // mostly L2 hits) one step before, either into registers (C loads issued first in the
     // step, so the store's wait for them leaves the younger A loads in flight) or, with DC,
-    // straight into the other LDS buffer.
+    // straight into the other LDS buffer (then the step's closing wait for the direct loads
+    // waits for every vector-memory load, and the A loads have one step to land).
     typedef v4u_t v4u;
     v4f_t a0[2 * NAV], a1[2 * NAV];  // 8 FP32 of one A row segment per vector, per set
     v4u c0h, c0l, c1h, c1l, c2h, c2l, c3h, c3l, c4h, c4l;
