@@ -73,6 +73,24 @@ void hcheck(hipError_t e, const char* what) {
     }
 }
 
+// A copy between device memory and pageable host memory (std::vector, caller arrays) that
+// neither overtakes the work queued on s nor outlives the call: s is drained first, the copy
+// is synchronous, and the device is idle afterwards, whatever the flags of s (a torch stream
+// does not synchronise with the legacy stream) and however the runtime stages pageable memory.
+// An asynchronous copy on s was seen to read a block of the metric before the upload queued
+// ahead of it had landed when kernel dispatches were traced.
+void copy_settled(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s, const char* what) {
+    hcheck(hipStreamSynchronize(s), what);
+    hcheck(hipMemcpy(dst, src, bytes, kind), what);
+    hcheck(hipDeviceSynchronize(), what);
+}
+void copy2d_settled(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows,
+                    hipMemcpyKind kind, hipStream_t s, const char* what) {
+    hcheck(hipStreamSynchronize(s), what);
+    hcheck(hipMemcpy2D(dst, dpitch, src, spitch, width, rows, kind), what);
+    hcheck(hipDeviceSynchronize(), what);
+}
+
 // Growable device buffer.
 struct DevBuf {
     void* p = nullptr;
@@ -356,6 +374,8 @@ void upload_tiles(dfjk_handle* h) {
     hcheck(hipMemcpy(h->d_otile_of_all.get<int>(of_all.size()), of_all.data(), of_all.size() * sizeof(int),
                      hipMemcpyHostToDevice),
            "upload");
+    // the kernels that read these tables run on h->stream, which need not wait for the legacy stream
+    hcheck(hipDeviceSynchronize(), "upload tile tables");
 }
 
 size_t b_doubles(const dfjk_handle* h) { return static_cast<size_t>(h->naux) * h->ntp * TILE2; }
@@ -470,7 +490,7 @@ std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<doubl
     if (dJm)
         hcheck(hipMemcpyAsync(A, dJm, nn * sizeof(double), hipMemcpyDeviceToDevice, s), "copy metric");
     else
-        hcheck(hipMemcpyAsync(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload metric");
+        copy_settled(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s, "upload metric");
     hcheck(hipMemsetAsync(X, 0, nn * sizeof(double), s), "memset");
     std::vector<double> blk(NB * NB), invT(NB * NB);
     std::vector<std::vector<double>> dinv;  // host inverses of the diagonal blocks
@@ -478,8 +498,7 @@ std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<doubl
         const int nb = std::min(NB, n - k0), k1 = k0 + nb, m = n - k1;
         const size_t bw = static_cast<size_t>(nb) * sizeof(double);
         double* Akk = A + static_cast<size_t>(k0) * n + k0;
-        hcheck(hipMemcpy2DAsync(blk.data(), bw, Akk, ld, bw, nb, hipMemcpyDeviceToHost, s), "diagonal block");
-        hcheck(hipStreamSynchronize(s), "diagonal block");
+        copy2d_settled(blk.data(), bw, Akk, ld, bw, nb, hipMemcpyDeviceToHost, s, "diagonal block");
         const int bad = cholesky_lower(blk.data(), nb, h->nthreads);
         if (bad >= 0) cholesky_failed(k0 + bad);
         dinv.emplace_back(static_cast<size_t>(nb) * nb);
@@ -487,12 +506,11 @@ std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<doubl
         lower_inverse(blk.data(), nb, inv.data(), h->nthreads);
         for (int r = 0; r < nb; ++r)
             for (int c = 0; c < nb; ++c) invT[static_cast<size_t>(c) * nb + r] = inv[static_cast<size_t>(r) * nb + c];
-        hcheck(hipMemcpy2DAsync(Akk, ld, blk.data(), bw, bw, nb, hipMemcpyHostToDevice, s), "L_kk");
-        hcheck(hipMemcpy2DAsync(X + static_cast<size_t>(k0) * n + k0, ld, inv.data(), bw, bw, nb,
-                                hipMemcpyHostToDevice, s),
-               "X_kk");
+        copy2d_settled(Akk, ld, blk.data(), bw, bw, nb, hipMemcpyHostToDevice, s, "L_kk");
+        copy2d_settled(X + static_cast<size_t>(k0) * n + k0, ld, inv.data(), bw, bw, nb, hipMemcpyHostToDevice, s,
+                       "X_kk");
         if (m > 0) {
-            hcheck(hipMemcpyAsync(Li, invT.data(), bw * nb, hipMemcpyHostToDevice, s), "L_kk^-T");
+            copy_settled(Li, invT.data(), bw * nb, hipMemcpyHostToDevice, s, "L_kk^-T");
             double* Aik = A + static_cast<size_t>(k1) * n + k0;
             launch_gemm_nn_f64(Aik, n, Li, nb, P, nb, m, nb, nb, 1.0, 0.0, s);
             hcheck(hipMemcpy2DAsync(Aik, ld, P, bw, bw, m, hipMemcpyDeviceToDevice, s), "panel");
@@ -505,9 +523,8 @@ std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<doubl
     for (int i0 = NB, bi = 1; i0 < n; i0 += NB, ++bi) {
         const int nb = std::min(NB, n - i0);
         launch_gemm_nn_f64(A + static_cast<size_t>(i0) * n, n, X, n, T, i0, nb, i0, i0, 1.0, 0.0, s);
-        hcheck(hipMemcpyAsync(Li, dinv[bi].data(), static_cast<size_t>(nb) * nb * sizeof(double),
-                              hipMemcpyHostToDevice, s),
-               "L_ii^-1");
+        copy_settled(Li, dinv[bi].data(), static_cast<size_t>(nb) * nb * sizeof(double), hipMemcpyHostToDevice, s,
+                     "L_ii^-1");
         launch_gemm_nn_f64(Li, nb, T, i0, X + static_cast<size_t>(i0) * n, n, nb, i0, nb, -1.0, 0.0, s, 1, 0, 0, 0,
                            1);
         hcheck(hipStreamSynchronize(s), "block row");
@@ -516,8 +533,7 @@ std::vector<double> device_metric_factor(dfjk_handle* h, const std::vector<doubl
     std::vector<double> M;
     if (want_host) {
         M.resize(nn);
-        hcheck(hipMemcpyAsync(M.data(), X, nn * sizeof(double), hipMemcpyDeviceToHost, s), "download M");
-        hcheck(hipStreamSynchronize(s), "download M");
+        copy_settled(M.data(), X, nn * sizeof(double), hipMemcpyDeviceToHost, s, "download M");
     }
     return M;
 }
@@ -542,7 +558,7 @@ std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double
     if (dJm)
         hcheck(hipMemcpyAsync(A, dJm, nn * sizeof(double), hipMemcpyDeviceToDevice, s), "copy metric");
     else
-        hcheck(hipMemcpyAsync(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload metric");
+        copy_settled(A, Jm.data(), nn * sizeof(double), hipMemcpyHostToDevice, s, "upload metric");
     rocblas_handle rb = nullptr;
     if (rocblas_create_handle(&rb) != rocblas_status_success) fail(DFJK_ERR_DEVICE, "rocblas_create_handle failed");
     struct RbGuard {
@@ -556,9 +572,8 @@ std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double
         fail(DFJK_ERR_DEVICE, std::string("rocsolver_dsyevd: ") + rocblas_status_to_string(st));
     std::vector<double> a(n);
     rocblas_int hinfo = 0;
-    hcheck(hipMemcpyAsync(a.data(), w, n * sizeof(double), hipMemcpyDeviceToHost, s), "eigenvalues");
-    hcheck(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s), "eigensolver status");
-    hcheck(hipStreamSynchronize(s), "metric eigensolver");
+    copy_settled(a.data(), w, n * sizeof(double), hipMemcpyDeviceToHost, s, "eigenvalues");
+    copy_settled(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s, "eigensolver status");
     if (hinfo != 0) fail(DFJK_ERR_INTERNAL, "metric eigensolver did not converge (info " + std::to_string(hinfo) + ")");
     // eigenvalues ascend: the largest magnitude is at one end
     const double max_a = std::max(std::fabs(a[n - 1]), std::fabs(a[0]));
@@ -575,7 +590,7 @@ std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double
     }
     h->fit_kept = kept;
     double* fd = df.get<double>(n);
-    hcheck(hipMemcpyAsync(fd, f.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "upload powers");
+    copy_settled(fd, f.data(), n * sizeof(double), hipMemcpyHostToDevice, s, "upload powers");
     // column-major eigenvectors read row-major: row i of A is eigenvector i (R).  M = R^T diag(f) R.
     double* S = dS.get<double>(nn);
     double* RT = dRT.get<double>(nn);
@@ -587,7 +602,7 @@ std::vector<double> device_metric_eigen(dfjk_handle* h, const std::vector<double
     std::vector<double> M;
     if (want_host) {
         M.resize(nn);
-        hcheck(hipMemcpyAsync(M.data(), X, nn * sizeof(double), hipMemcpyDeviceToHost, s), "download M");
+        copy_settled(M.data(), X, nn * sizeof(double), hipMemcpyDeviceToHost, s, "download M");
     }
     // the temporaries are freed on return: the products must have read them
     hcheck(hipStreamSynchronize(s), "metric power");
@@ -643,7 +658,7 @@ void fit_on_device(dfjk_handle* h, const std::vector<double>& M, const std::vect
     const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
     double* dM = h->ws_G.get<double>(static_cast<size_t>(na) * na);
     if (!h->dM_ready)
-        hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
+        copy_settled(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice, h->stream, "upload metric factor");
     // chunk of columns so that the staging buffer stays <= ~1 GiB
     size_t cw = std::max<size_t>(TILE2, (size_t(1) << 27) / std::max(1, na));
     cw = std::min(ncolB, cw / TILE2 * TILE2);
@@ -720,6 +735,8 @@ void upload_int_tables(const Basis& pb, const Basis* ab, IntTables& t, ThreeInde
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
         b.get<char>(std::max<size_t>(bytes, 8));
         if (bytes) hcheck(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "upload integral tables");
+        // read by kernels on the handle's stream, which need not wait for the legacy stream
+        hcheck(hipDeviceSynchronize(), "upload integral tables");
     };
     up(t.ps, ps.data(), ps.size() * sizeof(GpuShell));
     up(t.as, as.data(), as.size() * sizeof(GpuShell));
@@ -781,7 +798,7 @@ bool gpu_schwarz(dfjk_handle* h, double* fun_max) {
     upload_int_tables(pb, nullptr, tabs, a);
     DevBuf d_pairs, d_out;
     d_pairs.get<int2>(std::max<size_t>(1, pall.size()));
-    hcheck(hipMemcpy(d_pairs.p, pall.data(), pall.size() * sizeof(int2), hipMemcpyHostToDevice), "upload pairs");
+    copy_settled(d_pairs.p, pall.data(), pall.size() * sizeof(int2), hipMemcpyHostToDevice, h->stream, "upload pairs");
     const size_t nn = static_cast<size_t>(pb.nbf) * pb.nbf;
     double* out = d_out.get<double>(nn);
     hcheck(hipMemsetAsync(out, 0, nn * sizeof(double), h->stream), "memset schwarz");
@@ -793,8 +810,7 @@ bool gpu_schwarz(dfjk_handle* h, double* fun_max) {
             if (npair) launch_schwarz_class(a, la, lb, dp + poff[c], npair, out, h->stream);
         }
     hcheck(hipGetLastError(), "schwarz launch");
-    hcheck(hipMemcpyAsync(fun_max, out, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "schwarz download");
-    hcheck(hipStreamSynchronize(h->stream), "schwarz");
+    copy_settled(fun_max, out, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream, "schwarz download");
     return true;
 }
 
@@ -821,7 +837,7 @@ void run_three_index(dfjk_handle* h, const Basis& pb, const Basis& ab, const std
     DevBuf d_dst, d_pairs, d_afun;
     auto up = [&](DevBuf& b, const void* src, size_t bytes) {
         b.get<char>(bytes);
-        hcheck(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "upload integral tables");
+        copy_settled(b.p, src, bytes, hipMemcpyHostToDevice, h->stream, "upload integral tables");
     };
     if (dst) up(d_dst, dst->data(), dst->size() * sizeof(int64_t));
     up(d_pairs, pall.data(), std::max<size_t>(1, pall.size()) * sizeof(int2));
@@ -916,7 +932,7 @@ void fit_device_A(dfjk_handle* h, const std::vector<double>& M, const double* A,
     const size_t ncolB = static_cast<size_t>(h->ntp) * TILE2;
     double* dM = h->ws_G.get<double>(static_cast<size_t>(na) * na);
     if (!h->dM_ready)
-        hcheck(hipMemcpy(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice), "upload metric factor");
+        copy_settled(dM, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice, h->stream, "upload metric factor");
     if (!host_dst) {
         double* dBp = dst.get<double>(b_doubles(h));
         size_t cw = std::max<size_t>(TILE2, (size_t(1) << 30) / (sizeof(double) * std::max(1, na)));
@@ -1787,19 +1803,18 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
         double *gD = dD, *gJ = dJ, *gK = dK;
         if (!device) {
             double* raw = h->ws_craw.get<double>(2 * csz);
-            hcheck(hipMemcpyAsync(raw, cl, csz * sizeof(double), hipMemcpyHostToDevice, h->stream), "upload C_left");
+            copy_settled(raw, cl, csz * sizeof(double), hipMemcpyHostToDevice, h->stream, "upload C_left");
             cl = raw;
             if (cr) {
-                hcheck(hipMemcpyAsync(raw + csz, cr, csz * sizeof(double), hipMemcpyHostToDevice, h->stream),
-                       "upload C_right");
+                copy_settled(raw + csz, cr, csz * sizeof(double), hipMemcpyHostToDevice, h->stream, "upload C_right");
                 cr = raw + csz;
             }
             gD = dD ? h->ws_D.get<double>(nn) : nullptr;
             gJ = dJ ? h->ws_J.get<double>(nn) : nullptr;
             gK = dK ? h->ws_K.get<double>(nn) : nullptr;
             if (accumulate) {
-                if (gJ) hcheck(hipMemcpyAsync(gJ, dJ, nn * sizeof(double), hipMemcpyHostToDevice, h->stream), "up");
-                if (gK) hcheck(hipMemcpyAsync(gK, dK, nn * sizeof(double), hipMemcpyHostToDevice, h->stream), "up");
+                if (gJ) copy_settled(gJ, dJ, nn * sizeof(double), hipMemcpyHostToDevice, h->stream, "up");
+                if (gK) copy_settled(gK, dK, nn * sizeof(double), hipMemcpyHostToDevice, h->stream, "up");
             }
         }
         Work wk;  // per pair: the pairs run in stream order, so a fixed arena is reused
@@ -1807,11 +1822,11 @@ void compute_impl(dfjk_handle* h, int n, const double* const* Cl, const double* 
         wk.cap = arena_bytes;
         compute_pair(h, cl, cr, ncol[i], gD, gJ, gK, accumulate, wk);
         if (!device) {
-            if (dD) hcheck(hipMemcpyAsync(dD, gD, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download D");
+            if (dD) copy_settled(dD, gD, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream, "download D");
             if (dJ && h->do_J)
-                hcheck(hipMemcpyAsync(dJ, gJ, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download J");
+                copy_settled(dJ, gJ, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream, "download J");
             if (dK && h->do_K)
-                hcheck(hipMemcpyAsync(dK, gK, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download K");
+                copy_settled(dK, gK, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream, "download K");
             hcheck(hipStreamSynchronize(h->stream), "compute");
         }
     }
@@ -1854,21 +1869,20 @@ void compute_wk_impl(dfjk_handle* h, int n, const double* const* Cl, const doubl
         double* gW = dW;
         if (!device) {
             double* raw = h->ws_craw.get<double>(2 * csz);
-            hcheck(hipMemcpyAsync(raw, cl, csz * sizeof(double), hipMemcpyHostToDevice, h->stream), "upload C_left");
+            copy_settled(raw, cl, csz * sizeof(double), hipMemcpyHostToDevice, h->stream, "upload C_left");
             cl = raw;
             if (cr) {
-                hcheck(hipMemcpyAsync(raw + csz, cr, csz * sizeof(double), hipMemcpyHostToDevice, h->stream),
-                       "upload C_right");
+                copy_settled(raw + csz, cr, csz * sizeof(double), hipMemcpyHostToDevice, h->stream, "upload C_right");
                 cr = raw + csz;
             }
             gW = h->ws_K.get<double>(nn);
-            if (accumulate) hcheck(hipMemcpyAsync(gW, dW, nn * sizeof(double), hipMemcpyHostToDevice, h->stream), "up");
+            if (accumulate) copy_settled(gW, dW, nn * sizeof(double), hipMemcpyHostToDevice, h->stream, "up");
         }
         compute_wk_pair(h, cl, cr, ncol[i], gW, accumulate);
         // lr_symmetric: MemDFJK::compute_JK hermitivitizes wK (MemDFJK.cc:101-107)
         if (!cr) launch_hermitivitize(gW, h->nbf, h->nbf, h->stream);
         if (!device) {
-            hcheck(hipMemcpyAsync(dW, gW, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download wK");
+            copy_settled(dW, gW, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream, "download wK");
             hcheck(hipStreamSynchronize(h->stream), "compute wK");
         }
     }
@@ -1919,8 +1933,7 @@ void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double*
         double* part = h->ws_red.get<double>(kRed);
         launch_dot_partial(D, G, nn, part, kRed, s);
         std::vector<double> hp(kRed);
-        hcheck(hipMemcpyAsync(hp.data(), part, kRed * sizeof(double), hipMemcpyDeviceToHost, s), "energy");
-        hcheck(hipStreamSynchronize(s), "energy");
+        copy_settled(hp.data(), part, kRed * sizeof(double), hipMemcpyDeviceToHost, s, "energy");
         double e = 0.0;
         for (double v : hp) e += v;
         *E = e;
@@ -1963,10 +1976,8 @@ void adjoint_impl(dfjk_handle* h, const double* dC, int nocc, double* E, double*
         launch_fill_scaled(dvec, qa, nq, 4.0, D, nbf, out, s);
         launch_gemm_nn_f64(P, nocc, CT, nbf, out, nbf, nbf, nbf, nocc, -2.0, 1.0, s, nq, pq, 0, nn);
         if (gam_host) {
-            hcheck(hipMemcpyAsync(gam_host + static_cast<size_t>(qa - uq0) * nn, Gb, nn * nq * sizeof(double),
-                                  hipMemcpyDeviceToHost, s),
-                   "download dE/dB");
-            hcheck(hipStreamSynchronize(s), "adjoint");
+            copy_settled(gam_host + static_cast<size_t>(qa - uq0) * nn, Gb, nn * nq * sizeof(double),
+                         hipMemcpyDeviceToHost, s, "download dE/dB");
         }
     }
 }
@@ -1981,10 +1992,10 @@ void adjoint_entry(dfjk_handle* h, const double* C, int nocc, double* E, double*
         hcheck(hipStreamSynchronize(h->stream), "adjoint");
     } else {
         double* dc = h->ws_craw.get<double>(csz);
-        hcheck(hipMemcpyAsync(dc, C, csz * sizeof(double), hipMemcpyHostToDevice, h->stream), "upload C");
+        copy_settled(dc, C, csz * sizeof(double), hipMemcpyHostToDevice, h->stream, "upload C");
         double* g = dEdC ? h->ws_dc.get<double>(csz) : nullptr;
         adjoint_impl(h, dc, nocc, E, g, nullptr, dEdB);
-        if (dEdC) hcheck(hipMemcpyAsync(dEdC, g, csz * sizeof(double), hipMemcpyDeviceToHost, h->stream), "dE/dC");
+        if (dEdC) copy_settled(dEdC, g, csz * sizeof(double), hipMemcpyDeviceToHost, h->stream, "dE/dC");
         hcheck(hipStreamSynchronize(h->stream), "adjoint");
     }
     finish_timers(h);
@@ -2310,9 +2321,8 @@ void set_dense(dfjk_handle* h, const double* B, int on_device, DevBuf& dst) {
         double* stage = h->ws_XL.get<double>(nn * qc);
         for (int q0 = 0; q0 < h->naux; q0 += qc) {
             const int nq = std::min(h->naux - q0, qc);
-            hcheck(hipMemcpyAsync(stage, B + static_cast<size_t>(q0) * nn, nn * nq * sizeof(double),
-                                  hipMemcpyHostToDevice, s),
-                   "upload B");
+            copy_settled(stage, B + static_cast<size_t>(q0) * nn, nn * nq * sizeof(double), hipMemcpyHostToDevice, s,
+                         "upload B");
             launch_pack_dense(g, stage, h->d_mask.as<uint8_t>(), q0, nq, out, s);
         }
     }
@@ -2333,8 +2343,8 @@ void get_dense(dfjk_handle* h, const DevBuf& src, int q0, int q1, double* out) {
         if (host)
             std::memcpy(packed.data(), h->hB + Q * qs, qs * sizeof(double));
         else
-            hcheck(hipMemcpy(packed.data(), src.as<double>() + Q * qs, qs * sizeof(double), hipMemcpyDeviceToHost),
-                   "download B");
+            copy_settled(packed.data(), src.as<double>() + Q * qs, qs * sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream, "download B");
         double* o = out + (Q - q0) * nn;
         std::fill(o, o + nn, 0.0);
         for (int t = 0; t < h->ntp; ++t) {
@@ -2377,7 +2387,7 @@ int dfjk_fill_synthetic(dfjk_handle* h, uint64_t seed) {
         std::vector<double> decay(h->nbf);
         for (int k = 0; k < h->nbf; ++k) decay[k] = std::exp(-static_cast<double>(k) / 64.0);
         double* dd = h->ws_d.get<double>(std::max(h->nbf, h->naux));
-        hcheck(hipMemcpy(dd, decay.data(), decay.size() * sizeof(double), hipMemcpyHostToDevice), "upload decay");
+        copy_settled(dd, decay.data(), decay.size() * sizeof(double), hipMemcpyHostToDevice, h->stream, "upload decay");
         const TileGeom g = h->geom();
         for (int q0 = 0; q0 < h->naux; q0 += 32768) {
             const int q1 = std::min(h->naux, q0 + 32768);
