@@ -184,3 +184,31 @@ def test_device_schwarz_matches_host(gpu_available, case):
     assert np.array_equal(md, mh) and nd == nh
     if case == "ar5-dz":
         assert nd == 1962
+
+
+def test_setup_on_a_torch_stream_matches_own_stream(gpu_available):
+    """Set-up (Schwarz diagonal, metric, its factor, three-index integrals, fit) on a torch
+    stream, which does not synchronise with the legacy stream, gives the B of the handle's own
+    stream: every copy between device and pageable host memory waits for the work queued
+    before it and is complete when the call returns."""
+    import torch
+    from psi4_amd.basisset import BasisSet
+    from psi4_amd.core import JK
+    mol = M.water_cluster((2, 2, 1), seed=5)
+    prim = BasisSet.build(mol, "cc-pvdz")
+    aux = BasisSet.build(mol, "cc-pvdz-jkfit")
+
+    def build(stream):
+        jk = JK.build(prim, aux, memory=8_000_000_000)
+        jk.set_print(0)
+        if stream is not None:
+            jk.set_stream(stream.cuda_stream)
+        jk.initialize()
+        torch.cuda.synchronize()
+        return jk.get_B_dense(0, jk.info()["naux"])
+
+    announce("set-up on a torch stream vs the handle's own stream, (H2O)_4 cc-pVDZ")
+    B_own = build(None)
+    B_torch = build(torch.cuda.Stream())
+    assert np.abs(B_own).max() > 0
+    assert _rel(B_torch, B_own) < 1e-14
